@@ -1,0 +1,663 @@
+// flr_capi.hip -- the extern "C" boundary of libflr_hip.so (declared in include/flr.h).
+//
+// Handles own their device tables: orbit tables are packed once at creation, Lagrange and TDI
+// tables are uploaded once and re-uploaded only when a drop-in call passes different contents.
+// No entry point allocates per call; the drop-in calls synchronise their stream at the end, as
+// the reference launchers do (LISAResponse.cu:420,704), and report latched device errors.
+#include "flr_internal.hpp"
+#include "../../include/flr.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace flr;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+    return fail(FLR_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define FLR_HIP(call)                                  \
+    do {                                               \
+        hipError_t _e = (call);                        \
+        if (_e != hipSuccess) return hip_fail(_e, #call); \
+    } while (0)
+
+// Copy n elements from a host or device pointer into a host vector.
+template <typename T>
+int fetch(const T *src, size_t n, std::vector<T> &dst, const char *name)
+{
+    if (n > 0 && src == nullptr) return fail(FLR_ERR_INVALID_ARGUMENT, std::string(name) + " is NULL");
+    dst.resize(n);
+    if (n == 0) return FLR_OK;
+    hipError_t e = hipMemcpy(dst.data(), src, n * sizeof(T), hipMemcpyDefault);
+    if (e != hipSuccess) return hip_fail(e, name);
+    return FLR_OK;
+}
+
+// Keep the caller's current device unchanged across a call.
+struct DeviceGuard {
+    int prev = -1;
+    bool ok = true;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) ok = hipSetDevice(dev) == hipSuccess;
+    }
+    ~DeviceGuard()
+    {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+const int kStdLinks[kNLinks] = {12, 23, 31, 13, 32, 21};
+
+}  // namespace
+
+struct flr_orbits {
+    double dt = 0.0;
+    int N = 0;
+    double armlength = 0.0;
+    int links[kNLinks] = {};
+    int sc_r[kNLinks] = {};
+    int sc_e[kNLinks] = {};
+    double *d_table = nullptr;
+    double max_radius = 0.0;
+    double max_ltt = 0.0;
+    int device = 0;
+
+    int row_of(int link) const
+    {
+        for (int r = 0; r < kNLinks; ++r)
+            if (links[r] == link) return r;
+        return -1;
+    }
+    OrbitView view() const { return OrbitView{d_table, dt, N}; }
+};
+
+struct flr_response {
+    flr_orbits *orbits = nullptr;
+    int device = 0;
+    int *d_err = nullptr;
+    bool exact_default = false;
+
+    // Lagrange tables
+    bool lag_set = false;
+    int order = 0;
+    int num_A = 0;
+    double deps = 0.0;
+    std::vector<double> A_host, E_host;
+    double2 *d_Apair = nullptr;
+    double2 *d_Aslope = nullptr;
+    int A_cap = 0;
+
+    // TDI tables
+    bool tdi_set = false;
+    std::vector<int> sig_starts, sig_lengths, sig_base, sig_comb, sig_chan;
+    std::vector<double> sig_signs;
+    TdiUnit *d_units = nullptr;
+    int *d_chain = nullptr;
+    int ch_begin[4] = {0, 0, 0, 0};
+    int num_units = 0;
+    int max_chain = 0;  // most delayed links in one unit
+
+    LagrangeView lagrange() const
+    {
+        LagrangeView lg{};
+        lg.Apair = d_Apair;
+        lg.Aslope = d_Aslope;
+        lg.deps = deps;
+        lg.inv_deps = 1.0 / deps;
+        lg.num_A = num_A;
+        lg.h = (order + 1) / 2;
+        for (int j = 0; j < kMaxHalf; ++j) lg.E[j] = j < (int)E_host.size() ? E_host[j] : 0.0;
+        return lg;
+    }
+};
+
+namespace {
+
+int set_lagrange_impl(flr_response *r, int order, const double *A_in, double deps, int num_A,
+                      const double *E_in)
+{
+    const int h = (order + 1) / 2;
+    if (order < 1 || h > kMaxHalf)
+        return fail(FLR_ERR_UNSUPPORTED, "order must be in [1, " + std::to_string(2 * kMaxHalf - 1) + "]");
+    if (num_A < 2) return fail(FLR_ERR_INVALID_ARGUMENT, "num_A must be >= 2");
+    if (!(deps > 0.0)) return fail(FLR_ERR_INVALID_ARGUMENT, "deps must be > 0");
+    std::vector<double> A, E;
+    int st = fetch(A_in, (size_t)num_A, A, "A_in");
+    if (st) return st;
+    st = fetch(E_in, (size_t)(h > 1 ? h - 1 : 0), E, "E_in");
+    if (st) return st;
+    if (r->lag_set && r->order == order && r->num_A == num_A && r->deps == deps && r->A_host == A &&
+        r->E_host == E)
+        return FLR_OK;
+    if (num_A - 1 > r->A_cap) {
+        if (r->d_Apair) (void)hipFree(r->d_Apair);
+        if (r->d_Aslope) (void)hipFree(r->d_Aslope);
+        r->d_Apair = r->d_Aslope = nullptr;
+        r->A_cap = 0;
+        FLR_HIP(hipMalloc(&r->d_Apair, sizeof(double2) * (num_A - 1)));
+        FLR_HIP(hipMalloc(&r->d_Aslope, sizeof(double2) * (num_A - 1)));
+        r->A_cap = num_A - 1;
+    }
+    std::vector<double2> pair(num_A - 1), slope(num_A - 1);
+    for (int k = 0; k + 1 < num_A; ++k) {
+        pair[k] = make_double2(A[k], A[k + 1]);
+        slope[k] = make_double2(A[k], A[k + 1] - A[k]);
+    }
+    FLR_HIP(hipMemcpy(r->d_Apair, pair.data(), sizeof(double2) * pair.size(), hipMemcpyHostToDevice));
+    FLR_HIP(hipMemcpy(r->d_Aslope, slope.data(), sizeof(double2) * slope.size(), hipMemcpyHostToDevice));
+    r->order = order;
+    r->num_A = num_A;
+    r->deps = deps;
+    r->A_host = std::move(A);
+    r->E_host = std::move(E);
+    r->lag_set = true;
+    return FLR_OK;
+}
+
+int set_tdi_impl(flr_response *r, const int32_t *unit_starts, const int32_t *unit_lengths,
+                 const int32_t *tdi_base_link, const int32_t *tdi_link_combinations,
+                 const double *tdi_signs_in, const int32_t *channels, int num_units, int num_channels)
+{
+    if (num_units < 1 || num_units > kMaxUnits)
+        return fail(FLR_ERR_UNSUPPORTED, "num_units must be in [1, " + std::to_string(kMaxUnits) + "]");
+    if (num_channels < 1 || num_channels > 3)
+        return fail(FLR_ERR_INVALID_ARGUMENT, "num_channels must be in [1, 3]");
+    std::vector<int> starts, lengths, base, chan, comb;
+    std::vector<double> signs;
+    int st;
+    if ((st = fetch(unit_starts, num_units, starts, "unit_starts"))) return st;
+    if ((st = fetch(unit_lengths, num_units, lengths, "unit_lengths"))) return st;
+    if ((st = fetch(tdi_base_link, num_units, base, "tdi_base_link"))) return st;
+    if ((st = fetch(channels, num_units, chan, "channels"))) return st;
+    if ((st = fetch(tdi_signs_in, num_units, signs, "tdi_signs_in"))) return st;
+    int ncomb = 0;
+    for (int u = 0; u < num_units; ++u) {
+        if (starts[u] < 0 || lengths[u] < 0)
+            return fail(FLR_ERR_INVALID_ARGUMENT, "unit_starts/unit_lengths must be >= 0");
+        ncomb = std::max(ncomb, starts[u] + lengths[u]);
+    }
+    if (ncomb > kMaxChain)
+        return fail(FLR_ERR_UNSUPPORTED, "more than " + std::to_string(kMaxChain) + " delay-chain entries");
+    if ((st = fetch(tdi_link_combinations, ncomb, comb, "tdi_link_combinations"))) return st;
+
+    if (r->tdi_set && starts == r->sig_starts && lengths == r->sig_lengths && base == r->sig_base &&
+        chan == r->sig_chan && comb == r->sig_comb && signs == r->sig_signs)
+        return FLR_OK;
+
+    // Units of a channel keep their index order (the order of the reference's `+=`); channels
+    // are laid out one after the other.
+    std::vector<int> order_idx(num_units);
+    for (int u = 0; u < num_units; ++u) order_idx[u] = u;
+    for (int u = 0; u < num_units; ++u)
+        if (chan[u] < 0 || chan[u] >= num_channels || chan[u] > 2)
+            return fail(FLR_ERR_INVALID_ARGUMENT, "channel index out of range");
+    std::stable_sort(order_idx.begin(), order_idx.end(), [&](int a, int b) { return chan[a] < chan[b]; });
+
+    std::vector<TdiUnit> units;
+    std::vector<int> rows;
+    int max_chain = 0;
+    for (int u : order_idx) {
+        TdiUnit un{};
+        un.sign = signs[u];
+        un.base_row = r->orbits->row_of(base[u]);
+        if (un.base_row < 0)
+            return fail(FLR_ERR_INVALID_ARGUMENT, "unknown TDI base link " + std::to_string(base[u]));
+        un.chain_start = (int)rows.size();
+        un.chain_len = lengths[u];
+        int delayed = 0;
+        for (int s = 0; s < lengths[u]; ++s) {
+            const int l = comb[starts[u] + s];
+            if (l == -11) {
+                rows.push_back(-1);
+                continue;
+            }
+            const int row = r->orbits->row_of(l);
+            if (row < 0) return fail(FLR_ERR_INVALID_ARGUMENT, "unknown TDI delay link " + std::to_string(l));
+            rows.push_back(row);
+            ++delayed;
+        }
+        max_chain = std::max(max_chain, delayed);
+        units.push_back(un);
+    }
+    if (rows.size() > (size_t)kMaxChain)
+        return fail(FLR_ERR_UNSUPPORTED, "more than " + std::to_string(kMaxChain) + " delay-chain entries");
+    int ch_begin[4] = {0, 0, 0, 0};
+    for (int c = 0; c < 3; ++c) {
+        int cnt = 0;
+        for (int u = 0; u < num_units; ++u) cnt += chan[u] == c;
+        ch_begin[c + 1] = ch_begin[c] + cnt;
+    }
+    if (!r->d_units) FLR_HIP(hipMalloc(&r->d_units, sizeof(TdiUnit) * kMaxUnits));
+    if (!r->d_chain) FLR_HIP(hipMalloc(&r->d_chain, sizeof(int) * kMaxChain));
+    FLR_HIP(hipMemcpy(r->d_units, units.data(), sizeof(TdiUnit) * units.size(), hipMemcpyHostToDevice));
+    if (!rows.empty())
+        FLR_HIP(hipMemcpy(r->d_chain, rows.data(), sizeof(int) * rows.size(), hipMemcpyHostToDevice));
+    std::memcpy(r->ch_begin, ch_begin, sizeof(ch_begin));
+    r->num_units = num_units;
+    r->max_chain = max_chain;
+    r->sig_starts = std::move(starts);
+    r->sig_lengths = std::move(lengths);
+    r->sig_base = std::move(base);
+    r->sig_chan = std::move(chan);
+    r->sig_comb = std::move(comb);
+    r->sig_signs = std::move(signs);
+    r->tdi_set = true;
+    return FLR_OK;
+}
+
+constexpr size_t kMaxLds = 160 * 1024;
+
+int projection_impl(flr_response *r, double *y_gw, const double *input_in, int num_delays,
+                    int num_inputs, const double *t_data, double t0, double dt, double fs,
+                    const double *k, const double *u, const double *v, int start, int flags,
+                    hipStream_t stream)
+{
+    if (!r->lag_set) return fail(FLR_ERR_INVALID_ARGUMENT, "Lagrange tables not set");
+    if (!y_gw || !input_in) return fail(FLR_ERR_INVALID_ARGUMENT, "y_gw and input_in must not be NULL");
+    if (!(flags & FLR_FLAG_T_UNIFORM) && !t_data)
+        return fail(FLR_ERR_INVALID_ARGUMENT, "t_data is NULL without FLR_FLAG_T_UNIFORM");
+    if (num_delays < 1 || num_inputs < 1) return fail(FLR_ERR_INVALID_ARGUMENT, "empty input");
+    if (start < 0) return fail(FLR_ERR_INVALID_ARGUMENT, "projections_start_ind must be >= 0");
+    if (!(fs > 0.0)) return fail(FLR_ERR_INVALID_ARGUMENT, "sampling_frequency must be > 0");
+    const flr_orbits *o = r->orbits;
+    const int h = (r->order + 1) / 2;
+
+    ProjParams p{};
+    p.y = y_gw;
+    p.h = reinterpret_cast<const double2 *>(input_in);
+    p.t_data = (flags & FLR_FLAG_T_UNIFORM) ? nullptr : t_data;
+    p.num_delays = num_delays;
+    p.num_inputs = num_inputs;
+    p.start = start;
+    p.zero_edges = (flags & FLR_FLAG_ZERO_EDGES) ? 1 : 0;
+    p.grid_begin = p.zero_edges ? 0 : start;
+    p.t0 = t0;
+    p.dt = dt;
+    p.fs = fs;
+    for (int c = 0; c < 3; ++c) {
+        p.k[c] = k[c];
+        p.u[c] = u[c];
+        p.v[c] = v[c];
+    }
+    for (int s = 0; s < 3; ++s) {
+        int q = 0;
+        for (int row = 0; row < kNLinks; ++row) {
+            if (o->sc_r[row] != s + 1) continue;
+            if (q == 2) return fail(FLR_ERR_INVALID_ARGUMENT, "a spacecraft receives more than two links");
+            p.rec_row[s][q] = row;
+            p.em_sc[s][q] = o->sc_e[row] - 1;
+            ++q;
+        }
+        if (q != 2) return fail(FLR_ERR_INVALID_ARGUMENT, "every spacecraft must receive two links");
+    }
+    const double rc = o->max_radius * kCInv;
+    p.halo_lo = (int)std::ceil((o->max_ltt + rc) * fs) + h + 4;
+    p.halo_hi = (int)std::ceil(rc * fs) + h + 4;
+    p.win_cap = kBlock + p.halo_lo + p.halo_hi;
+    if ((size_t)p.win_cap * sizeof(double2) > kMaxLds)
+        return fail(FLR_ERR_UNSUPPORTED, "projection delay window exceeds LDS (sampling too fine)");
+    p.err = r->d_err;
+    hipError_t e = launch_projection(p, o->view(), r->lagrange(), (flags & FLR_FLAG_EXACT) != 0, stream);
+    if (e != hipSuccess) return hip_fail(e, "projection launch");
+    return FLR_OK;
+}
+
+int tdi_impl(flr_response *r, double *out, const double *y_gw, int num_inputs, int num_delays,
+             const double *t_arr, double t0, double dt, double fs, int start, int flags,
+             hipStream_t stream)
+{
+    if (!r->lag_set) return fail(FLR_ERR_INVALID_ARGUMENT, "Lagrange tables not set");
+    if (!r->tdi_set) return fail(FLR_ERR_INVALID_ARGUMENT, "TDI tables not set");
+    if (!out || !y_gw) return fail(FLR_ERR_INVALID_ARGUMENT, "output and y_gw must not be NULL");
+    if (!(flags & FLR_FLAG_T_UNIFORM) && !t_arr)
+        return fail(FLR_ERR_INVALID_ARGUMENT, "t_arr is NULL without FLR_FLAG_T_UNIFORM");
+    if (num_delays < 1 || num_inputs < 1) return fail(FLR_ERR_INVALID_ARGUMENT, "empty input");
+    if (start < 0) return fail(FLR_ERR_INVALID_ARGUMENT, "tdi_start_ind must be >= 0");
+    if (!(fs > 0.0)) return fail(FLR_ERR_INVALID_ARGUMENT, "sampling_frequency must be > 0");
+    const flr_orbits *o = r->orbits;
+    const int h = (r->order + 1) / 2;
+
+    TdiParams p{};
+    p.out = out;
+    p.y = y_gw;
+    p.t_arr = (flags & FLR_FLAG_T_UNIFORM) ? nullptr : t_arr;
+    p.num_inputs = num_inputs;
+    p.num_delays = num_delays;
+    p.start = start;
+    p.zero_edges = (flags & FLR_FLAG_ZERO_EDGES) ? 1 : 0;
+    p.grid_begin = p.zero_edges ? 0 : start;
+    p.t0 = t0;
+    p.dt = dt;
+    p.fs = fs;
+    p.units = r->d_units;
+    p.chain_rows = r->d_chain;
+    for (int c = 0; c < 4; ++c) p.ch_begin[c] = r->ch_begin[c];
+    p.halo_lo = (int)std::ceil(r->max_chain * o->max_ltt * fs) + h + 4;
+    p.halo_hi = h + 4;
+    p.win_cap = kBlock + p.halo_lo + p.halo_hi;
+    if (((size_t)kNLinks * p.win_cap + (size_t)kNLinks * kBlock) * sizeof(double) > kMaxLds)
+        return fail(FLR_ERR_UNSUPPORTED, "TDI delay window exceeds LDS (sampling too fine)");
+    p.aet = (flags & FLR_FLAG_AET) ? 1 : 0;
+    p.overwrite = ((flags & FLR_FLAG_OVERWRITE) || p.aet) ? 1 : 0;
+    p.s2 = std::sqrt(2.0);
+    p.s6 = std::sqrt(6.0);
+    p.s3 = std::sqrt(3.0);
+    p.err = r->d_err;
+    hipError_t e = launch_tdi(p, o->view(), r->lagrange(), (flags & FLR_FLAG_EXACT) != 0, stream);
+    if (e != hipSuccess) return hip_fail(e, "tdi launch");
+    return FLR_OK;
+}
+
+int check_errors_impl(flr_response *r, hipStream_t stream)
+{
+    FLR_HIP(hipStreamSynchronize(stream));
+    int word = 0;
+    FLR_HIP(hipMemcpy(&word, r->d_err, sizeof(int), hipMemcpyDeviceToHost));
+    if (word != 0) {
+        FLR_HIP(launch_clear(r->d_err, stream));
+        FLR_HIP(hipStreamSynchronize(stream));
+    }
+    if (word & kErrWindow)
+        return fail(FLR_ERR_WINDOW,
+                    "a stencil reached outside the staged sample window: the start index is too "
+                    "small for the delays, or the time grid is not uniform");
+    return FLR_OK;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+
+extern "C" {
+
+const char *flr_last_error(void) { return g_last_error.c_str(); }
+
+const char *flr_version(void) { return "lisa-on-gpu_amd 0.1.0 (gfx950)"; }
+
+int flr_device_count(int *count)
+{
+    if (!count) return fail(FLR_ERR_INVALID_ARGUMENT, "count is NULL");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return FLR_OK;
+}
+
+int flr_orbits_create(double dt, int N, const double *n_arr, const double *ltt_arr,
+                      const double *x_arr, const int32_t *links, const int32_t *sc_r,
+                      const int32_t *sc_e, double armlength, flr_orbits **out)
+{
+    if (!out) return fail(FLR_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    if (!(dt > 0.0)) return fail(FLR_ERR_INVALID_ARGUMENT, "dt must be > 0");
+    if (N < 2) return fail(FLR_ERR_INVALID_ARGUMENT, "N must be >= 2");
+    std::vector<double> n_h, ltt_h, x_h;
+    std::vector<int> links_h, scr_h, sce_h;
+    int st;
+    if ((st = fetch(n_arr, (size_t)N * 18, n_h, "n_arr"))) return st;
+    if ((st = fetch(ltt_arr, (size_t)N * 6, ltt_h, "ltt_arr"))) return st;
+    if ((st = fetch(x_arr, (size_t)N * 9, x_h, "x_arr"))) return st;
+    if ((st = fetch(links, 6, links_h, "links"))) return st;
+    if ((st = fetch(sc_r, 6, scr_h, "sc_r"))) return st;
+    if ((st = fetch(sc_e, 6, sce_h, "sc_e"))) return st;
+
+    auto o = new flr_orbits();
+    o->dt = dt;
+    o->N = N;
+    o->armlength = armlength;
+    bool seen[kNLinks] = {};
+    for (int r = 0; r < kNLinks; ++r) {
+        o->links[r] = links_h[r];
+        o->sc_r[r] = scr_h[r];
+        o->sc_e[r] = sce_h[r];
+        int which = -1;
+        for (int q = 0; q < kNLinks; ++q)
+            if (kStdLinks[q] == links_h[r]) which = q;
+        if (which < 0 || seen[which] || scr_h[r] != links_h[r] / 10 || sce_h[r] != links_h[r] % 10) {
+            delete o;
+            return fail(FLR_ERR_INVALID_ARGUMENT,
+                        "links must be a permutation of 12,23,31,13,32,21 with sc_r = link/10 and "
+                        "sc_e = link%10");
+        }
+        seen[which] = true;
+    }
+    std::vector<double> table((size_t)N * kOrbitStride, 0.0);
+    double rmax = 0.0, lmax = 0.0;
+    for (int w = 0; w < N; ++w) {
+        double *rec = &table[(size_t)w * kOrbitStride];
+        for (int r = 0; r < kNLinks; ++r) {
+            rec[r] = ltt_h[(size_t)w * 6 + r];
+            lmax = std::max(lmax, std::fabs(rec[r]));
+        }
+        for (int s = 0; s < 3; ++s) {
+            double r2 = 0.0;
+            for (int c = 0; c < 3; ++c) {
+                const double xv = x_h[((size_t)w * 3 + s) * 3 + c];
+                rec[6 + 3 * s + c] = xv;
+                r2 += xv * xv;
+            }
+            rmax = std::max(rmax, std::sqrt(r2));
+        }
+    }
+    for (int w = 0; w + 1 < N; ++w) {
+        double *rec = &table[(size_t)w * kOrbitStride];
+        const double *nxt = &table[(size_t)(w + 1) * kOrbitStride];
+        for (int v = 0; v < kOrbitVals; ++v) rec[kOrbitVals + v] = (nxt[v] - rec[v]) / dt;
+    }
+    if (!std::isfinite(rmax) || !std::isfinite(lmax)) {
+        delete o;
+        return fail(FLR_ERR_INVALID_ARGUMENT, "orbit tables hold non-finite values");
+    }
+    o->max_radius = rmax;
+    o->max_ltt = lmax;
+    hipError_t e = hipGetDevice(&o->device);
+    if (e == hipSuccess) e = hipMalloc(&o->d_table, table.size() * sizeof(double));
+    if (e == hipSuccess)
+        e = hipMemcpy(o->d_table, table.data(), table.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (o->d_table) (void)hipFree(o->d_table);
+        delete o;
+        return hip_fail(e, "orbit table upload");
+    }
+    *out = o;
+    return FLR_OK;
+}
+
+void flr_orbits_destroy(flr_orbits *orbits)
+{
+    if (!orbits) return;
+    DeviceGuard g(orbits->device);
+    if (orbits->d_table) (void)hipFree(orbits->d_table);
+    delete orbits;
+}
+
+int flr_orbits_extent(const flr_orbits *orbits, double *max_radius, double *max_ltt)
+{
+    if (!orbits) return fail(FLR_ERR_NO_ORBITS, "orbits is NULL");
+    if (max_radius) *max_radius = orbits->max_radius;
+    if (max_ltt) *max_ltt = orbits->max_ltt;
+    return FLR_OK;
+}
+
+int flr_orbits_eval(const flr_orbits *orbits, const double *t, int n, int link, int sc,
+                    double *ltt_out, double *pos_out, void *stream)
+{
+    if (!orbits) return fail(FLR_ERR_NO_ORBITS, "orbits is NULL");
+    if (n < 0 || (n > 0 && !t)) return fail(FLR_ERR_INVALID_ARGUMENT, "bad t array");
+    const int row = orbits->row_of(link);
+    if (ltt_out && row < 0) return fail(FLR_ERR_INVALID_ARGUMENT, "unknown link " + std::to_string(link));
+    if (pos_out && (sc < 1 || sc > 3)) return fail(FLR_ERR_INVALID_ARGUMENT, "sc must be 1, 2 or 3");
+    DeviceGuard g(orbits->device);
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e = launch_orbit_eval(orbits->view(), t, n, row < 0 ? 0 : row, sc - 1, ltt_out, pos_out, s);
+    if (e != hipSuccess) return hip_fail(e, "orbit eval launch");
+    return FLR_OK;
+}
+
+int flr_response_create(flr_orbits *orbits, flr_response **out)
+{
+    if (!out) return fail(FLR_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    if (!orbits) return fail(FLR_ERR_NO_ORBITS, "Must add orbits with add_orbit_information method.");
+    DeviceGuard g(orbits->device);
+    auto r = new flr_response();
+    r->orbits = orbits;
+    r->device = orbits->device;
+    hipError_t e = hipMalloc(&r->d_err, sizeof(int));
+    if (e == hipSuccess) {
+        const int zero = 0;
+        e = hipMemcpy(r->d_err, &zero, sizeof(int), hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        if (r->d_err) (void)hipFree(r->d_err);
+        delete r;
+        return hip_fail(e, "response create");
+    }
+    *out = r;
+    return FLR_OK;
+}
+
+void flr_response_destroy(flr_response *r)
+{
+    if (!r) return;
+    DeviceGuard g(r->device);
+    if (r->d_err) (void)hipFree(r->d_err);
+    if (r->d_Apair) (void)hipFree(r->d_Apair);
+    if (r->d_Aslope) (void)hipFree(r->d_Aslope);
+    if (r->d_units) (void)hipFree(r->d_units);
+    if (r->d_chain) (void)hipFree(r->d_chain);
+    delete r;
+}
+
+int flr_set_lagrange(flr_response *r, int order, const double *A_in, double deps, int num_A,
+                     const double *E_in)
+{
+    if (!r) return fail(FLR_ERR_INVALID_ARGUMENT, "response is NULL");
+    DeviceGuard g(r->device);
+    return set_lagrange_impl(r, order, A_in, deps, num_A, E_in);
+}
+
+int flr_set_tdi(flr_response *r, const int32_t *unit_starts, const int32_t *unit_lengths,
+                const int32_t *tdi_base_link, const int32_t *tdi_link_combinations,
+                const double *tdi_signs_in, const int32_t *channels, int num_units, int num_channels)
+{
+    if (!r) return fail(FLR_ERR_INVALID_ARGUMENT, "response is NULL");
+    if (!r->orbits) return fail(FLR_ERR_NO_ORBITS, "Must add orbits with add_orbit_information method.");
+    DeviceGuard g(r->device);
+    return set_tdi_impl(r, unit_starts, unit_lengths, tdi_base_link, tdi_link_combinations,
+                        tdi_signs_in, channels, num_units, num_channels);
+}
+
+int flr_projection(flr_response *r, double *y_gw, const double *input_in, int num_pts,
+                   const double *t_data, double t0, double dt, double sampling_frequency,
+                   const double *k, const double *u, const double *v, int projections_start_ind,
+                   int flags, void *stream)
+{
+    if (!r) return fail(FLR_ERR_INVALID_ARGUMENT, "response is NULL");
+    if (!r->orbits) return fail(FLR_ERR_NO_ORBITS, "Must add orbits with add_orbit_information method.");
+    if (!k || !u || !v) return fail(FLR_ERR_INVALID_ARGUMENT, "k, u, v must not be NULL");
+    DeviceGuard g(r->device);
+    return projection_impl(r, y_gw, input_in, num_pts, num_pts, t_data, t0, dt, sampling_frequency,
+                           k, u, v, projections_start_ind, flags, (hipStream_t)stream);
+}
+
+int flr_tdi(flr_response *r, double *out, const double *y_gw, int num_pts, const double *t_arr,
+            double t0, double dt, double sampling_frequency, int tdi_start_ind, int flags,
+            void *stream)
+{
+    if (!r) return fail(FLR_ERR_INVALID_ARGUMENT, "response is NULL");
+    if (!r->orbits) return fail(FLR_ERR_NO_ORBITS, "Must add orbits with add_orbit_information method.");
+    DeviceGuard g(r->device);
+    return tdi_impl(r, out, y_gw, num_pts, num_pts, t_arr, t0, dt, sampling_frequency, tdi_start_ind,
+                    flags, (hipStream_t)stream);
+}
+
+int flr_check_errors(flr_response *r, void *stream)
+{
+    if (!r) return fail(FLR_ERR_INVALID_ARGUMENT, "response is NULL");
+    DeviceGuard g(r->device);
+    return check_errors_impl(r, (hipStream_t)stream);
+}
+
+int flr_get_response(flr_response *r, double *y_gw, const double *t_data, const double *k_in,
+                     const double *u_in, const double *v_in, double dt, int num_delays,
+                     const double *input_in, int num_inputs, int order, double sampling_frequency,
+                     int buffer_integer, const double *A_in, double deps, int num_A,
+                     const double *E_in, int projections_start_ind, void *stream)
+{
+    (void)buffer_integer;
+    if (!r || !r->orbits) return fail(FLR_ERR_NO_ORBITS, "Must add orbits with add_orbit_information method.");
+    DeviceGuard g(r->device);
+    hipStream_t s = (hipStream_t)stream;
+    FLR_HIP(hipStreamSynchronize(s));  // the small tables may be produced on this stream
+    std::vector<double> k, u, v;
+    int st;
+    if ((st = fetch(k_in, 3, k, "k_in"))) return st;
+    if ((st = fetch(u_in, 3, u, "u_in"))) return st;
+    if ((st = fetch(v_in, 3, v, "v_in"))) return st;
+    if ((st = set_lagrange_impl(r, order, A_in, deps, num_A, E_in))) return st;
+    if (num_inputs < num_delays)
+        return fail(FLR_ERR_INVALID_ARGUMENT, "num_inputs must be >= num_delays");
+    FLR_HIP(launch_clear(r->d_err, s));
+    const int flags = r->exact_default ? FLR_FLAG_EXACT : 0;
+    st = projection_impl(r, y_gw, input_in, num_delays, num_inputs, t_data, 0.0, dt,
+                         sampling_frequency, k.data(), u.data(), v.data(), projections_start_ind,
+                         flags, s);
+    if (st) return st;
+    return check_errors_impl(r, s);
+}
+
+int flr_get_tdi_delays(flr_response *r, double *delayed_links, const double *input_links,
+                       int num_inputs, int num_delays, const double *t_arr,
+                       const int32_t *unit_starts, const int32_t *unit_lengths,
+                       const int32_t *tdi_base_link, const int32_t *tdi_link_combinations,
+                       const double *tdi_signs_in, const int32_t *channels, int num_units,
+                       int num_channels, int order, double sampling_frequency, int buffer_integer,
+                       const double *A_in, double deps, int num_A, const double *E_in,
+                       int tdi_start_ind, void *stream)
+{
+    (void)buffer_integer;
+    if (!r || !r->orbits) return fail(FLR_ERR_NO_ORBITS, "Must add orbits with add_orbit_information method.");
+    DeviceGuard g(r->device);
+    hipStream_t s = (hipStream_t)stream;
+    FLR_HIP(hipStreamSynchronize(s));
+    int st;
+    if ((st = set_lagrange_impl(r, order, A_in, deps, num_A, E_in))) return st;
+    if ((st = set_tdi_impl(r, unit_starts, unit_lengths, tdi_base_link, tdi_link_combinations,
+                           tdi_signs_in, channels, num_units, num_channels)))
+        return st;
+    FLR_HIP(launch_clear(r->d_err, s));
+    const int flags = r->exact_default ? FLR_FLAG_EXACT : 0;
+    st = tdi_impl(r, delayed_links, input_links, num_inputs, num_delays, t_arr, 0.0,
+                  1.0 / sampling_frequency, sampling_frequency, tdi_start_ind, flags, s);
+    if (st) return st;
+    return check_errors_impl(r, s);
+}
+
+int flr_response_set_exact(flr_response *r, int exact)
+{
+    if (!r) return fail(FLR_ERR_INVALID_ARGUMENT, "response is NULL");
+    r->exact_default = exact != 0;
+    return FLR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,226 @@
+// flr_device.hpp -- device building blocks of the response kernels.
+//
+// Everything that feeds a time, a delay or the integer/fraction index split repeats the
+// reference CPU operation sequence exactly (this file is compiled with -ffp-contract=off, so
+// no multiply-add is fused behind our back); only the stencil sums have a second, faster form.
+#pragma once
+
+#include "flr_internal.hpp"
+
+namespace flr {
+
+// ------------------------------------------------------------------ orbit interpolation
+// Uniform-grid linear interpolation of the orbit tables (the lisatools Orbits methods called
+// at LISAResponse.cu:331,565,570,576): window k = int(t/dt), clamped to [0, N-2], value
+// slope_k * (t - dt*k) + v_k with slope_k = (v_{k+1} - v_k)/dt precomputed on the host.
+
+struct OrbitNode {
+    const double *p;  // node record: 15 values then 15 slopes
+    double tau;       // t - dt*k
+};
+
+__device__ __forceinline__ OrbitNode orbit_node(const OrbitView &o, double t)
+{
+    double q = t / o.dt;
+    int w;
+    if (!(q >= 0.0))
+        w = 0;
+    else if (q >= (double)(o.N - 1))
+        w = o.N - 2;
+    else
+        w = (int)q;
+    OrbitNode n;
+    n.p = o.table + (size_t)w * kOrbitStride;
+    n.tau = t - (o.dt * w);
+    return n;
+}
+
+// Column: 0..5 light travel time of table row r; 6 + 3*s + c position of spacecraft s (0..2).
+__device__ __forceinline__ double orbit_val(const OrbitNode &n, int col)
+{
+    return n.p[kOrbitVals + col] * n.tau + n.p[col];
+}
+
+__device__ __forceinline__ double dot3(const double *a, const double *b)
+{
+    double out = 0.0;
+    out += a[0] * b[0];
+    out += a[1] * b[1];
+    out += a[2] * b[2];
+    return out;
+}
+
+// Integer/fraction split of a delay (LISAResponse.cu:339-341, :616-622): the stencil sits at
+// fractional sample d + 1 - e, with e in [0, 1).
+__device__ __forceinline__ void split_delay(double delay, double t0_offset, double fs, int &d,
+                                            double &e)
+{
+    double clipped = delay - t0_offset;
+    double cf = clipped * fs;
+    int k = (int)ceil(cf) - 1;
+    d = k;
+    e = (1.0 + (double)k) - cf;
+}
+
+// ------------------------------------------------------------------ Lagrange prefactor A(e)
+// Linear interpolation in the 1001-point table of A(e) (response.py:279-298), indexed like
+// LISAResponse.cu:114-117. The index is clamped so that no e can read outside the table.
+__device__ __forceinline__ double lagrange_A_exact(const LagrangeView &lg, double e)
+{
+    int ind = (int)(e / lg.deps);
+    double frac = (e - ind * lg.deps) / lg.deps;
+    ind = ind < 0 ? 0 : (ind > lg.num_A - 2 ? lg.num_A - 2 : ind);
+    double2 a = lg.Apair[ind];
+    return a.x * (1. - frac) + a.y * frac;
+}
+
+// Same table, multiply instead of divide and one fused step: differs from the exact form by
+// rounding only (the table is continuous across segment ends).
+__device__ __forceinline__ double lagrange_A_fast(const LagrangeView &lg, double e)
+{
+    double s = e * lg.inv_deps;
+    int ind = (int)s;
+    ind = ind < 0 ? 0 : (ind > lg.num_A - 2 ? lg.num_A - 2 : ind);
+    double frac = s - (double)ind;
+    double2 a = lg.Aslope[ind];
+    return __builtin_fma(a.y, frac, a.x);
+}
+
+// ------------------------------------------------------------------ stencils
+// x points at sample d of a window; taps x[d-(h-1)] .. x[d+h] are read.
+//
+// Exact form: the reference's sequence, out = A(B x[d+1] + C x[d] + D S),
+//   S = sum_{j=1}^{h-1} E_{j-1} (x[d+1+j]/(j+e) + x[d-j]/(j+1-e)),  B = 1-e, C = e, D = e(1-e)
+// (LISAResponse.cu:110-147 real, :149-207 complex).
+//
+// Fast form: same S without per-tap divisions. Pairing tap j up and down,
+//   x_u/(j+e) + x_d/(j+1-e) = (x_u (j+1-e) + x_d (j+e)) / Q_j,  Q_j = (j+e)(j+1-e) = j(j+1) + D,
+// and S = T/P with the running product P = prod Q_j and T_j = T_{j-1} Q_j + E_{j-1} n_j P_{j-1},
+// so one division per stencil replaces 2(h-1).
+
+template <int H>
+__device__ __forceinline__ double stencil_exact(const double *x, double e, const LagrangeView &lg)
+{
+    const int h = H > 0 ? H : lg.h;
+    constexpr int kUnroll = H > 0 ? H : 1;
+    double A = lagrange_A_exact(lg, e);
+    double B = 1.0 - e;
+    double C = e;
+    double D = e * (1.0 - e);
+    double sum = 0.0;
+#pragma unroll kUnroll
+    for (int j = 1; j < h; j += 1) {
+        double E = lg.E[j - 1];
+        double F = j + e;
+        double G = j + (1 - e);
+        double up = x[1 + j];
+        double down = x[-j];
+        sum += E * (up / F + down / G);
+    }
+    return A * (B * x[1] + C * x[0] + D * sum);
+}
+
+template <int H>
+__device__ __forceinline__ double stencil_fast(const double *x, double e, const LagrangeView &lg)
+{
+    const int h = H > 0 ? H : lg.h;
+    constexpr int kUnroll = H > 0 ? H : 1;
+    double A = lagrange_A_fast(lg, e);
+    double B = 1.0 - e;
+    double D = e * B;
+    double T = 0.0;
+    double P = 1.0;
+#pragma unroll kUnroll
+    for (int j = 1; j < h; j += 1) {
+        double up = x[1 + j];
+        double down = x[-j];
+        double qu = (double)j + e;
+        double qd = (double)(j + 1) - e;
+        double Q = (double)(j * (j + 1)) + D;
+        double num = __builtin_fma(up, qd, down * qu);
+        T = __builtin_fma(T, Q, (lg.E[j - 1] * num) * P);
+        P = P * Q;
+    }
+    double S = T / P;
+    return A * __builtin_fma(D, S, __builtin_fma(B, x[1], e * x[0]));
+}
+
+template <int H>
+__device__ __forceinline__ double2 stencil_exact_c(const double2 *x, double e, const LagrangeView &lg)
+{
+    const int h = H > 0 ? H : lg.h;
+    constexpr int kUnroll = H > 0 ? H : 1;
+    double A = lagrange_A_exact(lg, e);
+    double B = 1.0 - e;
+    double C = e;
+    double D = e * (1.0 - e);
+    double sum_re = 0.0, sum_im = 0.0;
+#pragma unroll kUnroll
+    for (int j = 1; j < h; j += 1) {
+        double E = lg.E[j - 1];
+        double F = j + e;
+        double G = j + (1 - e);
+        double2 up = x[1 + j];
+        double2 down = x[-j];
+        sum_re += E * (up.x / F + down.x / G);
+        sum_im += E * (up.y / F + down.y / G);
+    }
+    double2 up = x[1];
+    double2 down = x[0];
+    double2 out;
+    out.x = A * (B * up.x + C * down.x + D * sum_re);
+    out.y = A * (B * up.y + C * down.y + D * sum_im);
+    return out;
+}
+
+template <int H>
+__device__ __forceinline__ double2 stencil_fast_c(const double2 *x, double e, const LagrangeView &lg)
+{
+    const int h = H > 0 ? H : lg.h;
+    constexpr int kUnroll = H > 0 ? H : 1;
+    double A = lagrange_A_fast(lg, e);
+    double B = 1.0 - e;
+    double D = e * B;
+    double Tr = 0.0, Ti = 0.0;
+    double P = 1.0;
+#pragma unroll kUnroll
+    for (int j = 1; j < h; j += 1) {
+        double2 up = x[1 + j];
+        double2 down = x[-j];
+        double qu = (double)j + e;
+        double qd = (double)(j + 1) - e;
+        double Q = (double)(j * (j + 1)) + D;
+        double EP = lg.E[j - 1] * P;
+        double nr = __builtin_fma(up.x, qd, down.x * qu);
+        double ni = __builtin_fma(up.y, qd, down.y * qu);
+        Tr = __builtin_fma(Tr, Q, nr * EP);
+        Ti = __builtin_fma(Ti, Q, ni * EP);
+        P = P * Q;
+    }
+    double invP = 1.0 / P;
+    double2 x1 = x[1], x0 = x[0];
+    double2 out;
+    out.x = A * __builtin_fma(D, Tr * invP, __builtin_fma(B, x1.x, e * x0.x));
+    out.y = A * __builtin_fma(D, Ti * invP, __builtin_fma(B, x1.y, e * x0.y));
+    return out;
+}
+
+template <bool EXACT, int H>
+__device__ __forceinline__ double stencil(const double *x, double e, const LagrangeView &lg)
+{
+    if constexpr (EXACT)
+        return stencil_exact<H>(x, e, lg);
+    else
+        return stencil_fast<H>(x, e, lg);
+}
+
+template <bool EXACT, int H>
+__device__ __forceinline__ double2 stencil_c(const double2 *x, double e, const LagrangeView &lg)
+{
+    if constexpr (EXACT)
+        return stencil_exact_c<H>(x, e, lg);
+    else
+        return stencil_fast_c<H>(x, e, lg);
+}
+
+}  // namespace flr

@@ -1,0 +1,98 @@
+// flr_internal.hpp -- parameter blocks shared by the HIP kernels and the C-ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace flr {
+
+constexpr double kCInv = 3.3356409519815204e-09;  // 1/c in s/m (LISAResponse.hh:7)
+constexpr int kNLinks = 6;
+constexpr int kMaxHalf = 20;      // half point count h = (order+1)/2, order <= 39 (reference MAX_ORDER 40)
+constexpr int kMaxUnits = 256;    // TDI units (signed, delayed link terms)
+constexpr int kMaxChain = 4096;   // total delay-chain entries over all units
+constexpr int kBlock = 256;       // threads per workgroup (4 wave64)
+
+// Orbit table, packed per node: 15 values [ltt row 0..5 | x sc1 xyz | sc2 xyz | sc3 xyz]
+// followed by their 15 slopes (v[k+1] - v[k]) / dt, so that one node's data sits in 240 B.
+constexpr int kOrbitVals = 15;
+constexpr int kOrbitStride = 2 * kOrbitVals;
+
+// Device error word bits.
+constexpr int kErrWindow = 1;     // stencil outside the staged window (or the input array)
+
+struct OrbitView {
+    const double *table;  // [N][kOrbitStride]
+    double dt;
+    int N;
+};
+
+struct LagrangeView {
+    const double2 *Apair;   // [num_A-1]: (A[k], A[k+1]) -- exact form
+    const double2 *Aslope;  // [num_A-1]: (A[k], A[k+1]-A[k]) -- fast form
+    double deps;
+    double inv_deps;
+    int num_A;
+    int h;                  // half point count
+    double E[kMaxHalf];     // E[j-1], j = 1..h-1
+};
+
+// Projection: each spacecraft s (0..2) receives two links; rec_row[s][q] is the table row
+// (and y_gw row) of the q-th link received by s and em_sc[s][q] its emitter spacecraft.
+struct ProjParams {
+    double *y;              // [6][num_delays]
+    const double2 *h;       // [num_inputs] h+ + i hx
+    const double *t_data;   // [num_delays] or nullptr (uniform grid)
+    int num_delays;
+    int num_inputs;
+    int start;              // projections_start_ind
+    int grid_begin;         // first sample covered by tile 0
+    double t0, dt, fs;      // uniform grid t_i = t0 + i*dt
+    double k[3], u[3], v[3];
+    int rec_row[3][2];
+    int em_sc[3][2];
+    int halo_lo, halo_hi;   // staged window = [first - halo_lo, last + halo_hi]
+    int win_cap;            // LDS window capacity (complex samples)
+    int zero_edges;
+    int *err;
+};
+
+struct TdiUnit {
+    double sign;
+    int base_row;           // y_gw row of the delayed link
+    int chain_start;        // into chain_rows
+    int chain_len;
+    int pad;
+};
+
+struct TdiParams {
+    double *out;            // [3][num_delays]
+    const double *y;        // [6][num_inputs]
+    const double *t_arr;    // [num_delays] or nullptr
+    int num_inputs;
+    int num_delays;
+    int start;              // tdi_start_ind
+    int grid_begin;
+    double t0, dt, fs;
+    const TdiUnit *units;   // sorted by channel, index order kept within a channel
+    const int *chain_rows;  // orbit-table rows, -1 for the undelayed sentinel
+    int ch_begin[4];        // units [ch_begin[c], ch_begin[c+1]) feed channel c
+    int halo_lo, halo_hi;
+    int win_cap;            // LDS window capacity per link row (doubles)
+    int overwrite;
+    int aet;
+    int zero_edges;
+    double s2, s6, s3;      // sqrt(2), sqrt(6), sqrt(3)
+    int *err;
+};
+
+// Launchers (flr_kernels.hip).
+hipError_t launch_projection(const ProjParams &p, const OrbitView &orb, const LagrangeView &lg,
+                             bool exact, hipStream_t stream);
+hipError_t launch_tdi(const TdiParams &p, const OrbitView &orb, const LagrangeView &lg, bool exact,
+                      hipStream_t stream);
+hipError_t launch_orbit_eval(const OrbitView &orb, const double *t, int n, int row, int sc,
+                             double *ltt_out, double *pos_out, hipStream_t stream);
+hipError_t launch_clear(int *word, hipStream_t stream);
+
+}  // namespace flr

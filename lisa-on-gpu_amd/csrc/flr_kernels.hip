@@ -1,0 +1,316 @@
+// flr_kernels.hip -- gfx950 kernels of the LISA time-domain response.
+//
+// projection_kernel: one workgroup per tile of kBlock consecutive samples, one sample per lane,
+//   all six links per lane. The complex h window of the tile (tile + delay halo) is staged once
+//   in LDS and shared by the 3 receiver and 6 emitter stencils of every lane (the reference
+//   restages it per link, LISAResponse.cu:628-655). Receiver stencils depend on the receiving
+//   spacecraft only, so each is computed once for its two links: 9 complex stencils per sample
+//   instead of 12, with bit-identical results.
+// tdi_kernel: same tiling; the 6 y_ij rows of the tile window are staged once in LDS; each lane
+//   walks every TDI unit of every channel for its sample, summing units of a channel in index
+//   order in registers (no atomics), then stores X, Y, Z or A, E, T.
+#include "flr_device.hpp"
+
+namespace flr {
+
+// ------------------------------------------------------------------ projection
+
+template <bool EXACT, int H>
+__global__ __launch_bounds__(kBlock) void projection_kernel(ProjParams p, OrbitView orb,
+                                                            LagrangeView lg)
+{
+    extern __shared__ double2 hwin[];
+    const int h = H > 0 ? H : lg.h;
+    const int tid = threadIdx.x;
+    const int i0 = p.grid_begin + (int)blockIdx.x * kBlock;
+    const int i = i0 + tid;
+    const int lo_valid = p.start;
+    const int hi_valid = p.num_delays - p.start;
+    const bool valid = (i >= lo_valid) && (i < hi_valid);
+    const bool in_range = (i >= 0) && (i < p.num_delays);
+    const int first = max(i0, lo_valid);
+    const int last = min(i0 + kBlock, hi_valid) - 1;
+
+    if (!valid && in_range && p.zero_edges) {
+#pragma unroll
+        for (int l = 0; l < kNLinks; ++l) p.y[(size_t)l * p.num_delays + i] = 0.0;
+    }
+    if (first > last) return;  // whole tile outside the computed range
+
+    const int win_lo = first - p.halo_lo;
+    const int win_len = last + p.halo_hi - win_lo + 1;
+    if (win_len > p.win_cap || win_lo < 0 || win_lo + win_len > p.num_inputs) {
+        if (valid) {
+            atomicOr(p.err, kErrWindow);
+            for (int l = 0; l < kNLinks; ++l) p.y[(size_t)l * p.num_delays + i] = __builtin_nan("");
+        }
+        return;
+    }
+    for (int jj = tid; jj < win_len; jj += kBlock) hwin[jj] = p.h[win_lo + jj];
+
+    // Geometry of this lane's sample (overlaps the window loads in flight).
+    const int ic = valid ? i : first;
+    const double t = p.t_data ? p.t_data[ic] : p.t0 + (double)ic * p.dt;
+    const double t0_offset = p.t_data ? p.t_data[0] : p.t0 + (double)0 * p.dt;
+
+    int d_rec[3];
+    double e_rec[3];
+    int d_em[3][2];
+    double e_em[3][2], xi_p[3][2], xi_c[3][2], pre[3][2];
+
+    const OrbitNode nr = orbit_node(orb, t);
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        double xr[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) xr[c] = orbit_val(nr, 6 + 3 * s + c);
+        const double kxr = dot3(p.k, xr);
+        split_delay(t - kxr * kCInv, t0_offset, p.fs, d_rec[s], e_rec[s]);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int row = p.rec_row[s][q];
+            const int se = p.em_sc[s][q];
+            const double L = orbit_val(nr, row);
+            const double t_em = t - L;
+            const OrbitNode ne = orbit_node(orb, t_em);
+            double xe[3], n[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) xe[c] = orbit_val(ne, 6 + 3 * se + c);
+            double norm = 0.0;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                double nt = xr[c] - xe[c];
+                n[c] = nt;
+                norm += nt * nt;
+            }
+            norm = sqrt(norm);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) n[c] = n[c] / norm;
+            const double un = dot3(p.u, n);
+            const double vn = dot3(p.v, n);
+            xi_p[s][q] = 0.5 * ((un * un) - (vn * vn));
+            xi_c[s][q] = un * vn;
+            const double kn = dot3(p.k, n);
+            const double kxe = dot3(p.k, xe);
+            pre[s][q] = 1. / (1. - kn);
+            split_delay(t_em - kxe * kCInv, t0_offset, p.fs, d_em[s][q], e_em[s][q]);
+        }
+    }
+    __syncthreads();
+    if (!valid) return;
+
+    bool bad = false;
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        const int lr = d_rec[s] - win_lo;
+        bad |= (lr - (h - 1) < 0) || (lr + h >= win_len);
+        double2 hr = make_double2(0.0, 0.0);
+        if (!bad) hr = stencil_c<EXACT, H>(hwin + lr, e_rec[s], lg);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int le = d_em[s][q] - win_lo;
+            const bool bq = bad || (le - (h - 1) < 0) || (le + h >= win_len);
+            double out = __builtin_nan("");
+            if (!bq) {
+                const double2 he = stencil_c<EXACT, H>(hwin + le, e_em[s][q], lg);
+                const double large = (he.x - hr.x) * xi_p[s][q] + (he.y - hr.y) * xi_c[s][q];
+                out = pre[s][q] * large;
+            } else {
+                atomicOr(p.err, kErrWindow);
+            }
+            p.y[(size_t)p.rec_row[s][q] * p.num_delays + i] = out;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ TDI
+
+template <bool EXACT, int H>
+__global__ __launch_bounds__(kBlock) void tdi_kernel(TdiParams p, OrbitView orb, LagrangeView lg)
+{
+    extern __shared__ double smem[];
+    const int h = H > 0 ? H : lg.h;
+    double *ywin = smem;                    // [6][win_cap]
+    double *Ls = smem + kNLinks * p.win_cap;  // [6][kBlock]
+    const int tid = threadIdx.x;
+    const int i0 = p.grid_begin + (int)blockIdx.x * kBlock;
+    const int i = i0 + tid;
+    const int lo_valid = p.start;
+    const int hi_valid = p.num_delays - p.start;
+    const bool valid = (i >= lo_valid) && (i < hi_valid);
+    const bool in_range = (i >= 0) && (i < p.num_delays);
+    const int first = max(i0, lo_valid);
+    const int last = min(i0 + kBlock, hi_valid) - 1;
+
+    if (!valid && in_range && p.zero_edges) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) p.out[(size_t)c * p.num_delays + i] = 0.0;
+    }
+    if (first > last) return;
+
+    const int win_lo = first - p.halo_lo;
+    const int win_len = last + p.halo_hi - win_lo + 1;
+    if (win_len > p.win_cap || win_lo < 0 || win_lo + win_len > p.num_inputs) {
+        if (valid) {
+            atomicOr(p.err, kErrWindow);
+            for (int c = 0; c < 3; ++c) p.out[(size_t)c * p.num_delays + i] = __builtin_nan("");
+        }
+        return;
+    }
+    for (int r = 0; r < kNLinks; ++r) {
+        const double *src = p.y + (size_t)r * p.num_inputs + win_lo;
+        double *dst = ywin + r * p.win_cap;
+        for (int jj = tid; jj < win_len; jj += kBlock) dst[jj] = src[jj];
+    }
+
+    const int ic = valid ? i : first;
+    const double t = p.t_arr ? p.t_arr[ic] : p.t0 + (double)ic * p.dt;
+    const double t0_offset = p.t_arr ? p.t_arr[0] : p.t0 + (double)0 * p.dt;
+    {
+        const OrbitNode nd = orbit_node(orb, t);
+#pragma unroll
+        for (int r = 0; r < kNLinks; ++r) Ls[r * kBlock + tid] = orbit_val(nd, r);
+    }
+    __syncthreads();
+    if (!valid) return;
+
+    double acc[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        acc[c] = (p.overwrite || p.aet) ? 0.0 : p.out[(size_t)c * p.num_delays + i];
+
+    bool bad = false;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        for (int u = p.ch_begin[c]; u < p.ch_begin[c + 1]; ++u) {
+            const TdiUnit un = p.units[u];
+            // Same-t delay chain: every light travel time is taken at the undelayed t and
+            // subtracted in chain order (LISAResponse.cu:314-333).
+            double delay = t;
+            for (int s = 0; s < un.chain_len; ++s) {
+                const int row = p.chain_rows[un.chain_start + s];
+                if (row >= 0) delay -= Ls[row * kBlock + tid];
+            }
+            int d;
+            double e;
+            split_delay(delay, t0_offset, p.fs, d, e);
+            const int ld = d - win_lo;
+            double val;
+            if ((ld - (h - 1) >= 0) && (ld + h < win_len)) {
+                val = stencil<EXACT, H>(ywin + un.base_row * p.win_cap + ld, e, lg);
+            } else {
+                bad = true;
+                val = __builtin_nan("");
+            }
+            val *= un.sign;
+            acc[c] += val;
+        }
+    }
+    if (bad) atomicOr(p.err, kErrWindow);
+
+    if (p.aet) {
+        const double X = acc[0], Y = acc[1], Z = acc[2];
+        p.out[i] = (Z - X) / p.s2;
+        p.out[(size_t)p.num_delays + i] = (X - 2.0 * Y + Z) / p.s6;
+        p.out[(size_t)2 * p.num_delays + i] = (X + Y + Z) / p.s3;
+    } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) p.out[(size_t)c * p.num_delays + i] = acc[c];
+    }
+}
+
+// ------------------------------------------------------------------ small kernels
+
+__global__ void orbit_eval_kernel(OrbitView orb, const double *t, int n, int row, int sc,
+                                  double *ltt_out, double *pos_out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const OrbitNode nd = orbit_node(orb, t[i]);
+    if (ltt_out) ltt_out[i] = orbit_val(nd, row);
+    if (pos_out) {
+        for (int c = 0; c < 3; ++c) pos_out[3 * i + c] = orbit_val(nd, 6 + 3 * sc + c);
+    }
+}
+
+__global__ void clear_kernel(int *word)
+{
+    if (threadIdx.x == 0) word[0] = 0;
+}
+
+// ------------------------------------------------------------------ launchers
+
+template <typename Kernel>
+static hipError_t set_lds_limit(Kernel k, size_t bytes)
+{
+    if (bytes <= 64 * 1024) return hipSuccess;
+    return hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+template <bool EXACT, int H>
+static hipError_t launch_projection_t(const ProjParams &p, const OrbitView &orb,
+                                      const LagrangeView &lg, int nblocks, size_t shmem,
+                                      hipStream_t stream)
+{
+    auto k = projection_kernel<EXACT, H>;
+    hipError_t err = set_lds_limit(k, shmem);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(k, dim3(nblocks), dim3(kBlock), shmem, stream, p, orb, lg);
+    return hipGetLastError();
+}
+
+hipError_t launch_projection(const ProjParams &p, const OrbitView &orb, const LagrangeView &lg,
+                             bool exact, hipStream_t stream)
+{
+    const int span = p.zero_edges ? p.num_delays : p.num_delays - 2 * p.start;
+    if (span <= 0) return hipSuccess;
+    const int nblocks = (span + kBlock - 1) / kBlock;
+    const size_t shmem = (size_t)p.win_cap * sizeof(double2);
+    if (lg.h == 13)
+        return exact ? launch_projection_t<true, 13>(p, orb, lg, nblocks, shmem, stream)
+                     : launch_projection_t<false, 13>(p, orb, lg, nblocks, shmem, stream);
+    return exact ? launch_projection_t<true, 0>(p, orb, lg, nblocks, shmem, stream)
+                 : launch_projection_t<false, 0>(p, orb, lg, nblocks, shmem, stream);
+}
+
+template <bool EXACT, int H>
+static hipError_t launch_tdi_t(const TdiParams &p, const OrbitView &orb, const LagrangeView &lg,
+                               int nblocks, size_t shmem, hipStream_t stream)
+{
+    auto k = tdi_kernel<EXACT, H>;
+    hipError_t err = set_lds_limit(k, shmem);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(k, dim3(nblocks), dim3(kBlock), shmem, stream, p, orb, lg);
+    return hipGetLastError();
+}
+
+hipError_t launch_tdi(const TdiParams &p, const OrbitView &orb, const LagrangeView &lg, bool exact,
+                      hipStream_t stream)
+{
+    const int span = p.zero_edges ? p.num_delays : p.num_delays - 2 * p.start;
+    if (span <= 0) return hipSuccess;
+    const int nblocks = (span + kBlock - 1) / kBlock;
+    const size_t shmem = ((size_t)kNLinks * p.win_cap + (size_t)kNLinks * kBlock) * sizeof(double);
+    if (lg.h == 13)
+        return exact ? launch_tdi_t<true, 13>(p, orb, lg, nblocks, shmem, stream)
+                     : launch_tdi_t<false, 13>(p, orb, lg, nblocks, shmem, stream);
+    return exact ? launch_tdi_t<true, 0>(p, orb, lg, nblocks, shmem, stream)
+                 : launch_tdi_t<false, 0>(p, orb, lg, nblocks, shmem, stream);
+}
+
+hipError_t launch_orbit_eval(const OrbitView &orb, const double *t, int n, int row, int sc,
+                             double *ltt_out, double *pos_out, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(orbit_eval_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, orb, t, n,
+                       row, sc, ltt_out, pos_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_clear(int *word, hipStream_t stream)
+{
+    hipLaunchKernelGGL(clear_kernel, dim3(1), dim3(64), 0, stream, word);
+    return hipGetLastError();
+}
+
+}  // namespace flr

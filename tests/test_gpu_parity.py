@@ -1,0 +1,323 @@
+"""Parity of the gfx950 kernels with the FP64 CPU oracle (run on the GPU).
+
+Exact form (reference operation order): bitwise equal to the oracle.
+Fast form (reciprocal-free stencils): relative L2 <= 1e-12 per channel, the north-star bound.
+Time/delay/index arithmetic is shared by both forms, so the fast form's only deviation is
+stencil rounding.
+"""
+import numpy as np
+import pytest
+
+from conftest import YRSID_SI, gb_waveform, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL_FAST = 1e-12
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import fastlisaresponse_amd as flr
+    from fastlisaresponse_amd import _capi
+    _capi.lib()
+    return flr
+
+
+@pytest.fixture(scope="module")
+def orbits():
+    from fastlisaresponse_amd.orbits import EqualArmlengthOrbits
+    o = EqualArmlengthOrbits(t_max=2.0 * YRSID_SI)
+    o.configure(linear_interp_setup=True)
+    return o
+
+
+@pytest.fixture(scope="module")
+def oracle_orbits(oracle_mod, orbits):
+    return oracle_mod.OracleOrbits(*orbits.pycppdetector_args)
+
+
+def _setup(flr, orbits, order=25, tdi="1st generation"):
+    from fastlisaresponse_amd.backend import LISAResponseWrap, OrbitsWrap
+    from fastlisaresponse_amd.tables import lagrange_tables, stock_tdi, tdi_tables
+    ow = OrbitsWrap(*orbits.pycppdetector_args)
+    rw = LISAResponseWrap(ow)
+    A, deps, E = lagrange_tables(order)
+    rw.set_lagrange(order, A, deps, len(A), E)
+    tabs = tdi_tables(stock_tdi(tdi) if isinstance(tdi, str) else tdi)
+    rw.set_tdi(tabs["unit_starts"], tabs["unit_lengths"], tabs["tdi_base_links"],
+               tabs["tdi_link_combinations"], tabs["tdi_signs"], tabs["channels"],
+               tabs["num_units"], 3)
+    return ow, rw, (A, deps, E), tabs
+
+
+def _oracle_tables(tabs):
+    return dict(unit_starts=tabs["unit_starts"], unit_lengths=tabs["unit_lengths"],
+                tdi_base_link=tabs["tdi_base_links"],
+                tdi_link_combinations=tabs["tdi_link_combinations"], tdi_signs=tabs["tdi_signs"],
+                channels=tabs["channels"], num_units=tabs["num_units"])
+
+
+# ------------------------------------------------------------------ orbits
+
+def test_orbit_interpolation_bitwise(gpu, orbits, oracle_orbits):
+    import torch
+    ow, _, _, _ = _setup(gpu, orbits)
+    rng = np.random.default_rng(11)
+    t = np.concatenate([rng.uniform(0, orbits.t[-1], 2000), orbits.t[:50], [orbits.t[-1]],
+                        [-5.0, orbits.t[-1] + 7.0]])
+    tt = torch.as_tensor(t, device="cuda")
+    for link in orbits.LINKS:
+        got = ow.light_travel_time(tt, link).cpu().numpy()
+        exp = oracle_orbits.ltt_at(t, link)
+        np.testing.assert_array_equal(got, exp)
+    for sc in (1, 2, 3):
+        got = ow.position(tt, sc).cpu().numpy()
+        exp = oracle_orbits.pos_at(t, sc)
+        np.testing.assert_array_equal(got, exp)
+
+
+# ------------------------------------------------------------------ projection
+
+@pytest.mark.parametrize("t0", [0.0, 3.0e7])
+@pytest.mark.parametrize("exact", [True, False])
+def test_projection_vs_oracle(gpu, orbits, oracle_orbits, oracle_mod, t0, exact):
+    import torch
+    from fastlisaresponse_amd import _capi
+    from fastlisaresponse_amd.tables import buffer_indices, sky_basis
+    _, rw, (A, deps, E), _ = _setup(gpu, orbits)
+    n, dt, order = 8192, 10.0, 25
+    fs = 1 / dt
+    h = gb_waveform(n, dt, f=5e-3, fdot=1e-16, t0=t0)
+    k, u, v = sky_basis(1.2, 0.3)
+    _, _, _, P = buffer_indices(fs, order, 10000.0, orbits.x_base)
+    t_data = t0 + np.arange(n) * dt
+    y_exp = oracle_mod.get_response(oracle_orbits, h, t_data, k, u, v, order, fs, A, deps, E, P, 8)
+    y = torch.full((6 * n,), 7.0, dtype=torch.float64, device="cuda")
+    flags = _capi.FLR_FLAG_ZERO_EDGES | (_capi.FLR_FLAG_EXACT if exact else 0)
+    rw.projection(y, torch.as_tensor(h, device="cuda"), n, t0, dt, fs, k, u, v, P, flags)
+    rw.check_errors()
+    y = y.cpu().numpy().reshape(6, n)
+    if exact:
+        np.testing.assert_array_equal(y, y_exp)
+    else:
+        np.testing.assert_array_equal(y[:, :P], 0.0)
+        np.testing.assert_array_equal(y[:, n - P:], 0.0)
+        for l in range(6):
+            assert rel_l2(y[l], y_exp[l]) <= TOL_FAST, (l, rel_l2(y[l], y_exp[l]))
+
+
+def test_projection_t_data_path_matches_uniform(gpu, orbits):
+    """Reading t_data gives the same bits as synthesising t0 + i*dt in-kernel."""
+    import torch
+    from fastlisaresponse_amd import _capi
+    from fastlisaresponse_amd.tables import sky_basis
+    _, rw, _, _ = _setup(gpu, orbits)
+    n, dt, t0 = 4096, 10.0, 1.234567e7
+    h = torch.as_tensor(gb_waveform(n, dt, t0=t0), device="cuda")
+    k, u, v = sky_basis(0.4, -0.7)
+    t_data = torch.as_tensor(t0 + np.arange(n) * dt, device="cuda")
+    y1 = torch.zeros(6 * n, dtype=torch.float64, device="cuda")
+    y2 = torch.zeros(6 * n, dtype=torch.float64, device="cuda")
+    rw.projection(y1, h, n, t0, dt, 1 / dt, k, u, v, 780, 0, t_data=t_data)
+    rw.projection(y2, h, n, t0, dt, 1 / dt, k, u, v, 780, 0)
+    rw.check_errors()
+    assert torch.equal(y1, y2)
+
+
+# ------------------------------------------------------------------ TDI
+
+@pytest.mark.parametrize("tdi", ["1st generation", "2nd generation"])
+@pytest.mark.parametrize("exact", [True, False])
+def test_tdi_vs_oracle(gpu, orbits, oracle_orbits, oracle_mod, tdi, exact):
+    import torch
+    from fastlisaresponse_amd import _capi
+    _, rw, (A, deps, E), tabs = _setup(gpu, orbits, tdi=tdi)
+    n, dt, order, T, t0 = 8192, 10.0, 25, 1000, 2.5e7
+    fs = 1 / dt
+    rng = np.random.default_rng(5)
+    # smooth projections: sums of sinusoids (a white-noise input would make the interpolant
+    # ill-conditioned and the tolerance meaningless)
+    tt = np.arange(n) * dt
+    y = np.zeros((6, n))
+    for l in range(6):
+        for _ in range(4):
+            y[l] += rng.normal() * np.sin(2 * np.pi * rng.uniform(1e-4, 1e-2) * tt + rng.uniform(0, 6))
+    t_arr = t0 + np.arange(n) * dt
+    xyz_exp = oracle_mod.get_tdi_delays(oracle_orbits, y, t_arr, _oracle_tables(tabs), order, fs,
+                                        A, deps, E, T, 8)
+    out = torch.zeros(3 * n, dtype=torch.float64, device="cuda")
+    flags = _capi.FLR_FLAG_OVERWRITE | _capi.FLR_FLAG_ZERO_EDGES
+    flags |= _capi.FLR_FLAG_EXACT if exact else 0
+    yd = torch.as_tensor(y.reshape(-1), device="cuda")
+    rw.tdi(out, yd, n, t0, dt, fs, T, flags)
+    rw.check_errors()
+    xyz = out.cpu().numpy().reshape(3, n)
+    if exact:
+        np.testing.assert_array_equal(xyz, xyz_exp)
+    else:
+        for c in range(3):
+            assert rel_l2(xyz[c], xyz_exp[c]) <= TOL_FAST, (c, rel_l2(xyz[c], xyz_exp[c]))
+    # fused AET epilogue: same arithmetic as the oracle's AET
+    aet = torch.zeros(3 * n, dtype=torch.float64, device="cuda")
+    rw.tdi(aet, yd, n, t0, dt, fs, T, flags | _capi.FLR_FLAG_AET)
+    rw.check_errors()
+    aet = aet.cpu().numpy().reshape(3, n)
+    aet_exp = np.stack(oracle_mod.aet(*xyz))
+    np.testing.assert_array_equal(aet, aet_exp)
+
+
+def test_tdi_custom_combination(gpu, orbits, oracle_orbits, oracle_mod):
+    """A user list of terms (Sagnac-like alpha) goes through the generic table path."""
+    import torch
+    from fastlisaresponse_amd import _capi
+    combo = [
+        {"link": 12, "links_for_delay": [], "sign": 1},
+        {"link": 23, "links_for_delay": [12], "sign": 1},
+        {"link": 31, "links_for_delay": [12, 23], "sign": 1},
+        {"link": 13, "links_for_delay": [], "sign": -1},
+        {"link": 32, "links_for_delay": [13], "sign": -1},
+        {"link": 21, "links_for_delay": [13, 32], "sign": -0.5},
+    ]
+    _, rw, (A, deps, E), tabs = _setup(gpu, orbits, tdi=combo)
+    n, dt, T = 6000, 10.0, 1000
+    rng = np.random.default_rng(9)
+    tt = np.arange(n) * dt
+    y = np.stack([np.cos(2 * np.pi * rng.uniform(1e-4, 5e-3) * tt + l) for l in range(6)])
+    xyz_exp = oracle_mod.get_tdi_delays(oracle_orbits, y, tt, _oracle_tables(tabs), 25, 0.1, A, deps,
+                                        E, T, 8)
+    out = torch.zeros(3 * n, dtype=torch.float64, device="cuda")
+    rw.tdi(out, torch.as_tensor(y.reshape(-1), device="cuda"), n, 0.0, dt, 0.1, T,
+           _capi.FLR_FLAG_OVERWRITE | _capi.FLR_FLAG_EXACT)
+    rw.check_errors()
+    np.testing.assert_array_equal(out.cpu().numpy().reshape(3, n), xyz_exp)
+
+
+# ------------------------------------------------------------------ drop-in entry points
+
+def test_dropin_wraps_match_oracle(gpu, orbits, oracle_orbits, oracle_mod):
+    """get_response_wrap / get_tdi_delays_wrap with the reference's argument lists, accumulate
+    semantics and t arrays, in the exact (bitwise) form."""
+    import torch
+    from fastlisaresponse_amd.tables import buffer_indices, sky_basis
+    _, rw, (A, deps, E), tabs = _setup(gpu, orbits, tdi="2nd generation")
+    rw.set_exact(True)
+    n, dt, order, t0 = 5000, 10.0, 25, 4.0e6
+    fs = 1 / dt
+    h = gb_waveform(n, dt, t0=t0)
+    k, u, v = sky_basis(2.0, -0.2)
+    T, _, _, P = buffer_indices(fs, order, 10000.0, orbits.x_base)
+    t_data = t0 + np.arange(n) * dt
+    y = torch.zeros(6 * n, dtype=torch.float64, device="cuda")
+    rw.get_response_wrap(y, torch.as_tensor(t_data, device="cuda"), k, u, v, dt, n,
+                         torch.as_tensor(h, device="cuda"), n, order, fs, 2 * order + 1, A, deps,
+                         len(A), E, P)
+    y_exp = oracle_mod.get_response(oracle_orbits, h, t_data, k, u, v, order, fs, A, deps, E, P, 8)
+    np.testing.assert_array_equal(y.cpu().numpy().reshape(6, n), y_exp)
+    pre = np.random.default_rng(1).normal(size=3 * n)  # the kernel adds into the buffer
+    dl = torch.as_tensor(pre.copy(), device="cuda")
+    rw.get_tdi_delays_wrap(dl, y, n, n, torch.as_tensor(t_data, device="cuda"),
+                           tabs["unit_starts"], tabs["unit_lengths"], tabs["tdi_base_links"],
+                           tabs["tdi_link_combinations"], tabs["tdi_signs"], tabs["channels"],
+                           tabs["num_units"], 3, order, fs, 2 * order + 1, A, deps, len(A), E, T)
+    exp = oracle_mod.get_tdi_delays(oracle_orbits, y_exp, t_data, _oracle_tables(tabs), order, fs,
+                                    A, deps, E, T, 8, delayed_links=pre)
+    np.testing.assert_array_equal(dl.cpu().numpy().reshape(3, n), exp)
+    # numpy outputs are written back in place
+    y_np = np.zeros(6 * n)
+    rw.get_response_wrap(y_np, t_data, k, u, v, dt, n, h, n, order, fs, 2 * order + 1, A, deps,
+                         len(A), E, P)
+    np.testing.assert_array_equal(y_np.reshape(6, n), y_exp)
+
+
+def test_dropin_length_checks(gpu, orbits):
+    import torch
+    _, rw, (A, deps, E), _ = _setup(gpu, orbits)
+    n = 3000
+    with pytest.raises(ValueError, match="y_gw: input arrays have the incorrect length"):
+        rw.get_response_wrap(torch.zeros(6 * n - 1, dtype=torch.float64, device="cuda"),
+                             np.arange(n) * 10.0, np.ones(3), np.ones(3), np.ones(3), 10.0, n,
+                             np.zeros(n, dtype=complex), n, 25, 0.1, 51, A, deps, len(A), E, 780)
+
+
+def test_window_error_is_reported_not_faulted(gpu, orbits):
+    """A start index too small for the delays: kernels clamp every access and latch an error."""
+    import torch
+    from fastlisaresponse_amd import _capi
+    from fastlisaresponse_amd.tables import sky_basis
+    _, rw, _, _ = _setup(gpu, orbits)
+    n, dt = 4096, 10.0
+    h = torch.as_tensor(gb_waveform(n, dt), device="cuda")
+    k, u, v = sky_basis(1.0, 0.5)
+    y = torch.zeros(6 * n, dtype=torch.float64, device="cuda")
+    rw.projection(y, h, n, 0.0, dt, 0.1, k, u, v, 5, 0)
+    with pytest.raises(_capi.FLRError, match="window"):
+        rw.check_errors()
+    out = torch.zeros(3 * n, dtype=torch.float64, device="cuda")
+    rw.tdi(out, y.nan_to_num(), n, 0.0, dt, 0.1, 3, _capi.FLR_FLAG_OVERWRITE)
+    with pytest.raises(_capi.FLRError, match="window"):
+        rw.check_errors()
+    rw.check_errors()  # latched word was cleared
+
+
+# ------------------------------------------------------------------ full API
+
+@pytest.mark.parametrize("tdi", ["1st generation", "2nd generation"])
+def test_response_wrapper_vs_oracle(gpu, orbits, oracle_orbits, oracle_mod, tdi):
+    """ResponseWrapper on a 2^17-sample galactic binary (the 15-day CPU-reference workload)."""
+    from fastlisaresponse_amd import ResponseWrapper
+
+    n, dt = 2 ** 17, 10.0
+
+    def gb(A, f, fdot, iota, phi0, psi, T=1.0, dt=10.0):
+        return gb_waveform(n, dt, A, f, fdot, iota, phi0, psi)
+
+    wrap = ResponseWrapper(gb, n * dt / YRSID_SI, dt, 6, 7, t_buffer=10000.0,
+                           remove_sky_coords=True, n_overide=n, orbits=orbits, order=25, tdi=tdi,
+                           tdi_chan="AET")
+    chans = wrap(1e-21, 2e-3, 1e-17, 0.8, 0.3, 0.5, 1.2, 0.3)
+    chans = np.stack([c.cpu().numpy() for c in chans])
+    h = gb_waveform(n, dt)
+    _, aet = oracle_mod.response_pipeline(oracle_orbits, h, 1.2, 0.3, dt, 25, tdi,
+                                          x_base=orbits.x_base, nthreads=8)
+    T = 1000
+    for c in range(3):
+        err = rel_l2(chans[c], aet[c, T:-T])
+        assert err <= TOL_FAST, (c, err)
+    assert np.all(np.isfinite(chans))
+    # the XYZ property recomputes X, Y, Z after a fused AET call
+    xyz = wrap.response_model.XYZ.cpu().numpy()
+    assert np.allclose(np.stack(oracle_mod.aet(*xyz))[:, T:-T], chans, rtol=0, atol=1e-12 * np.abs(chans).max())
+
+
+def test_response_wrapper_exact_is_bitwise(gpu, orbits, oracle_orbits, oracle_mod):
+    from fastlisaresponse_amd import ResponseWrapper
+    n, dt = 40000, 10.0
+
+    def gb(A, f, fdot, iota, phi0, psi, T=1.0, dt=10.0):
+        return gb_waveform(n, dt, A, f, fdot, iota, phi0, psi)
+
+    wrap = ResponseWrapper(gb, n * dt / YRSID_SI, dt, 6, 7, remove_sky_coords=True, n_overide=n,
+                           orbits=orbits, tdi="2nd generation", tdi_chan="XYZ", exact=True,
+                           remove_garbage="zero")
+    chans = np.stack([c.cpu().numpy() for c in wrap(2e-21, 4e-3, 1e-16, 0.3, 1.0, 2.0, 4.0, -1.0)])
+    h = gb_waveform(n, dt, 2e-21, 4e-3, 1e-16, 0.3, 1.0, 2.0)
+    y_exp, xyz = oracle_mod.response_pipeline(oracle_orbits, h, 4.0, -1.0, dt, 25, "2nd generation",
+                                              tdi_chan="XYZ", x_base=orbits.x_base, nthreads=8)
+    np.testing.assert_array_equal(wrap.response_model.y_gw.cpu().numpy(), y_exp)
+    np.testing.assert_array_equal(chans, xyz)
+
+
+def test_get_tdi_delays_with_given_projections(gpu, orbits):
+    import torch
+    from fastlisaresponse_amd import pyResponseTDI
+    n = 20000
+    r = pyResponseTDI(0.1, n, orbits=orbits, tdi="1st generation", tdi_chan="XYZ")
+    r.get_projections(gb_waveform(n, 10.0), 1.0, 0.2)
+    X1 = torch.stack(r.get_tdi_delays()).clone()
+    X2 = torch.stack(r.get_tdi_delays(y_gw=r.y_gw.clone()))
+    assert torch.equal(X1, X2)
+    with pytest.raises(ValueError):
+        r.get_tdi_delays(t0=5.0)
