@@ -86,7 +86,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--tdi", default="2nd generation", choices=list(FLOP_TDI))
-    ap.add_argument("--exact", action="store_true", help="reference stencil operation order")
+    ap.add_argument("--fast", action="store_true",
+                    help="reciprocal-free stencils instead of the (default) bitwise-exact form")
+    ap.add_argument("--orbits", default="equal", choices=["equal", "unequal"],
+                    help="equal-arm cartwheel (default) or the unequal-arm cartwheel")
     ap.add_argument("--cpu-sample", type=int, default=1 << 21,
                     help="samples in the CPU-baseline sample (0 skips it)")
     ap.add_argument("--no-parity", action="store_true", help="skip the full-size oracle check")
@@ -101,14 +104,15 @@ def main() -> int:
     import numpy as np
     import torch
     from fastlisaresponse_amd import pyResponseTDI
-    from fastlisaresponse_amd.orbits import EqualArmlengthOrbits
+    from fastlisaresponse_amd.orbits import CartwheelOrbits, EqualArmlengthOrbits
 
     torch.cuda.set_device(0)
-    orbits = EqualArmlengthOrbits(t_max=2.0 * YRSID_SI)
+    orbit_cls = EqualArmlengthOrbits if args.orbits == "equal" else CartwheelOrbits
+    orbits = orbit_cls(t_max=2.0 * YRSID_SI)
     orbits.configure(linear_interp_setup=True)
     lam, beta = 1.2, 0.3
     r = pyResponseTDI(1 / DT, N_SAMPLES, order=ORDER, tdi=args.tdi, orbits=orbits, tdi_chan="AET",
-                      exact=args.exact)
+                      exact=not args.fast)
     h_host = gb_waveform(N_SAMPLES, DT)
     h = torch.as_tensor(h_host, device="cuda")
 
@@ -158,7 +162,9 @@ def main() -> int:
         "data": "synthetic (one galactic binary, f0=5 mHz, fdot=1e-16; analytic equal-arm orbits)",
         "config": {"workload": f"1-yr order-25 {args.tdi} TDI AET, FP64 plain GPU form",
                    "num_pts": N_SAMPLES, "dt": DT, "order": ORDER, "tdi": args.tdi,
-                   "tdi_chan": "AET", "stencil_form": "exact" if args.exact else "fast"},
+                   "tdi_chan": "AET", "stencil_form": "fast" if args.fast else "exact",
+                   "orbits": "equal-arm cartwheel" if args.orbits == "equal"
+                   else "unequal-arm cartwheel"},
         "kernels_ms": {"projection_kernel": proj_ms, "tdi_kernel": tdi_ms},
         "roofline": roofline,
     }

@@ -152,7 +152,8 @@ int flr_tdi(flr_response *response, double *out, const double *y_gw, int num_pts
 int flr_check_errors(flr_response *response, void *stream);
 
 /* Stencil form used by the drop-in entry points flr_get_response / flr_get_tdi_delays:
- * 0 (default) the fast reciprocal-free form, nonzero the reference operation order. */
+ * nonzero (default) the reference operation order, bitwise equal to the FP64 CPU path;
+ * 0 the faster reciprocal-free form (equal to ~1e-15 relative per stencil). */
 int flr_response_set_exact(flr_response *response, int exact);
 
 #ifdef __cplusplus

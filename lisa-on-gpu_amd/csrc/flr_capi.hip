@@ -5,6 +5,7 @@
 // No entry point allocates per call; the drop-in calls synchronise their stream at the end, as
 // the reference launchers do (LISAResponse.cu:420,704), and report latched device errors.
 #include "flr_internal.hpp"
+#include "flr_tdi_stock.hpp"
 #include "../../include/flr.h"
 
 #include <algorithm>
@@ -79,6 +80,7 @@ struct flr_orbits {
     double *d_table = nullptr;
     double max_radius = 0.0;
     double max_ltt = 0.0;
+    bool equal_arm = false;  // all six light-travel-time columns bitwise equal at every node
     int device = 0;
 
     int row_of(int link) const
@@ -94,7 +96,7 @@ struct flr_response {
     flr_orbits *orbits = nullptr;
     int device = 0;
     int *d_err = nullptr;
-    bool exact_default = false;
+    bool exact_default = true;
 
     // Lagrange tables
     bool lag_set = false;
@@ -115,6 +117,7 @@ struct flr_response {
     int ch_begin[4] = {0, 0, 0, 0};
     int num_units = 0;
     int max_chain = 0;  // most delayed links in one unit
+    int stock_gen = 0;  // 1 or 2 when the table is the stock 1st/2nd-generation one
 
     LagrangeView lagrange() const
     {
@@ -171,6 +174,33 @@ int set_lagrange_impl(flr_response *r, int order, const double *A_in, double dep
     r->E_host = std::move(E);
     r->lag_set = true;
     return FLR_OK;
+}
+
+// True when the units, in channel-then-index order, are exactly the stock table of generation
+// `gen` (same links, signs, chains and channels), which tdi_equal_arm_kernel hard-codes.
+bool is_stock_table(int gen, const std::vector<int> &order_idx, const std::vector<int> &starts,
+                    const std::vector<int> &lengths, const std::vector<int> &base,
+                    const std::vector<int> &comb, const std::vector<double> &signs,
+                    const std::vector<int> &chan)
+{
+    const int nx = gen == 1 ? 8 : 16;
+    if ((int)order_idx.size() != 3 * nx) return false;
+    int k = 0;
+    for (int c = 0; c < 3; ++c)
+        for (int x = 0; x < nx; ++x, ++k) {
+            const int u = order_idx[k];
+            const stock::Term &tm = stock::kX[x];
+            if (chan[u] != c || base[u] != stock::cyc(tm.link, c) || signs[u] != (double)tm.sign)
+                return false;
+            if (tm.len == 0) {
+                if (lengths[u] != 1 || comb[starts[u]] != -11) return false;
+            } else {
+                if (lengths[u] != tm.len) return false;
+                for (int s = 0; s < tm.len; ++s)
+                    if (comb[starts[u] + s] != stock::cyc(tm.chain[s], c)) return false;
+            }
+        }
+    return true;
 }
 
 int set_tdi_impl(flr_response *r, const int32_t *unit_starts, const int32_t *unit_lengths,
@@ -254,6 +284,9 @@ int set_tdi_impl(flr_response *r, const int32_t *unit_starts, const int32_t *uni
     std::memcpy(r->ch_begin, ch_begin, sizeof(ch_begin));
     r->num_units = num_units;
     r->max_chain = max_chain;
+    r->stock_gen = 0;
+    for (int gen = 1; gen <= 2 && r->stock_gen == 0; ++gen)
+        if (is_stock_table(gen, order_idx, starts, lengths, base, comb, signs, chan)) r->stock_gen = gen;
     r->sig_starts = std::move(starts);
     r->sig_lengths = std::move(lengths);
     r->sig_base = std::move(base);
@@ -362,7 +395,12 @@ int tdi_impl(flr_response *r, double *out, const double *y_gw, int num_inputs, i
     p.s6 = std::sqrt(6.0);
     p.s3 = std::sqrt(3.0);
     p.err = r->d_err;
-    hipError_t e = launch_tdi(p, o->view(), r->lagrange(), (flags & FLR_FLAG_EXACT) != 0, stream);
+    StockTdi st{};
+    st.generation = r->stock_gen;
+    st.equal_arm = o->equal_arm;
+    const int canon_links[6] = {12, 23, 31, 13, 32, 21};
+    for (int k = 0; k < 6; ++k) st.rows.row[k] = o->row_of(canon_links[k]);
+    hipError_t e = launch_tdi(p, o->view(), r->lagrange(), (flags & FLR_FLAG_EXACT) != 0, st, stream);
     if (e != hipSuccess) return hip_fail(e, "tdi launch");
     return FLR_OK;
 }
@@ -469,6 +507,10 @@ int flr_orbits_create(double dt, int N, const double *n_arr, const double *ltt_a
     }
     o->max_radius = rmax;
     o->max_ltt = lmax;
+    o->equal_arm = true;
+    for (int w = 0; w < N && o->equal_arm; ++w)
+        for (int r = 1; r < kNLinks; ++r)
+            if (table[(size_t)w * kOrbitStride + r] != table[(size_t)w * kOrbitStride]) o->equal_arm = false;
     hipError_t e = hipGetDevice(&o->device);
     if (e == hipSuccess) e = hipMalloc(&o->d_table, table.size() * sizeof(double));
     if (e == hipSuccess)

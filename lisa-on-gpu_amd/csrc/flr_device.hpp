@@ -2,12 +2,40 @@
 //
 // Everything that feeds a time, a delay or the integer/fraction index split repeats the
 // reference CPU operation sequence exactly (this file is compiled with -ffp-contract=off, so
-// no multiply-add is fused behind our back); only the stencil sums have a second, faster form.
+// no multiply-add is fused behind our back). The stencils come in two forms:
+//   exact: the reference's operation sequence with correctly rounded divisions -- bitwise
+//          equal to the FP64 CPU path;
+//   fast:  the same interpolant without per-tap divisions, equal to ~1e-15 relative.
 #pragma once
 
 #include "flr_internal.hpp"
 
 namespace flr {
+
+// ------------------------------------------------------------------ division
+// gfx950 evaluates a / b as: v_rcp_f64 seed, two Newton steps y = y + y(1 - b y), quotient
+// q = a y, remainder r = a - b q (exact), corrected quotient q + r y, wrapped in
+// v_div_scale / v_div_fmas / v_div_fixup, which only act when an operand needs exponent
+// rescaling (|a| below 2^-969, denormal or huge operands) or is special. Splitting the
+// sequence at the reciprocal lets one y serve several numerators with the same denominator
+// while producing the same bits as `a / b` -- the correctly rounded quotient -- for operands
+// in the normal range, which signals and projections always are (|h| ~ 1e-21).
+
+__device__ __forceinline__ double rcp_nr2(double b)
+{
+    double y = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-b, y, 1.0);
+    return __builtin_fma(y, e, y);
+}
+
+__device__ __forceinline__ double div_rcp(double a, double b, double y)
+{
+    double q = a * y;
+    double r = __builtin_fma(-b, q, a);
+    return __builtin_fma(r, y, q);
+}
 
 // ------------------------------------------------------------------ orbit interpolation
 // Uniform-grid linear interpolation of the orbit tables (the lisatools Orbits methods called
@@ -64,11 +92,12 @@ __device__ __forceinline__ void split_delay(double delay, double t0_offset, doub
 
 // ------------------------------------------------------------------ Lagrange prefactor A(e)
 // Linear interpolation in the 1001-point table of A(e) (response.py:279-298), indexed like
-// LISAResponse.cu:114-117. The index is clamped so that no e can read outside the table.
-__device__ __forceinline__ double lagrange_A_exact(const LagrangeView &lg, double e)
+// LISAResponse.cu:114-117; y_deps = rcp_nr2(deps). The index is clamped so that no e can read
+// outside the table (the clamp never triggers for e in [0, 1)).
+__device__ __forceinline__ double lagrange_A_exact(const LagrangeView &lg, double e, double y_deps)
 {
-    int ind = (int)(e / lg.deps);
-    double frac = (e - ind * lg.deps) / lg.deps;
+    int ind = (int)div_rcp(e, lg.deps, y_deps);
+    double frac = div_rcp(e - ind * lg.deps, lg.deps, y_deps);
     ind = ind < 0 ? 0 : (ind > lg.num_A - 2 ? lg.num_A - 2 : ind);
     double2 a = lg.Apair[ind];
     return a.x * (1. - frac) + a.y * frac;
@@ -96,14 +125,15 @@ __device__ __forceinline__ double lagrange_A_fast(const LagrangeView &lg, double
 // Fast form: same S without per-tap divisions. Pairing tap j up and down,
 //   x_u/(j+e) + x_d/(j+1-e) = (x_u (j+1-e) + x_d (j+e)) / Q_j,  Q_j = (j+e)(j+1-e) = j(j+1) + D,
 // and S = T/P with the running product P = prod Q_j and T_j = T_{j-1} Q_j + E_{j-1} n_j P_{j-1},
-// so one division per stencil replaces 2(h-1).
+// so one reciprocal per stencil replaces 2(h-1) divisions.
 
 template <int H>
-__device__ __forceinline__ double stencil_exact(const double *x, double e, const LagrangeView &lg)
+__device__ __forceinline__ double stencil_exact(const double *x, double e, const LagrangeView &lg,
+                                                double y_deps)
 {
     const int h = H > 0 ? H : lg.h;
     constexpr int kUnroll = H > 0 ? H : 1;
-    double A = lagrange_A_exact(lg, e);
+    double A = lagrange_A_exact(lg, e, y_deps);
     double B = 1.0 - e;
     double C = e;
     double D = e * (1.0 - e);
@@ -115,7 +145,7 @@ __device__ __forceinline__ double stencil_exact(const double *x, double e, const
         double G = j + (1 - e);
         double up = x[1 + j];
         double down = x[-j];
-        sum += E * (up / F + down / G);
+        sum += E * (div_rcp(up, F, rcp_nr2(F)) + div_rcp(down, G, rcp_nr2(G)));
     }
     return A * (B * x[1] + C * x[0] + D * sum);
 }
@@ -138,19 +168,20 @@ __device__ __forceinline__ double stencil_fast(const double *x, double e, const 
         double qd = (double)(j + 1) - e;
         double Q = (double)(j * (j + 1)) + D;
         double num = __builtin_fma(up, qd, down * qu);
-        T = __builtin_fma(T, Q, (lg.E[j - 1] * num) * P);
+        T = __builtin_fma(T, Q, num * (lg.E[j - 1] * P));
         P = P * Q;
     }
-    double S = T / P;
-    return A * __builtin_fma(D, S, __builtin_fma(B, x[1], e * x[0]));
+    return A * __builtin_fma(D, T * (1.0 / P), __builtin_fma(B, x[1], e * x[0]));
 }
 
+// Complex: the h+ and hx sums share each denominator's reciprocal.
 template <int H>
-__device__ __forceinline__ double2 stencil_exact_c(const double2 *x, double e, const LagrangeView &lg)
+__device__ __forceinline__ double2 stencil_exact_c(const double2 *x, double e, const LagrangeView &lg,
+                                                   double y_deps)
 {
     const int h = H > 0 ? H : lg.h;
     constexpr int kUnroll = H > 0 ? H : 1;
-    double A = lagrange_A_exact(lg, e);
+    double A = lagrange_A_exact(lg, e, y_deps);
     double B = 1.0 - e;
     double C = e;
     double D = e * (1.0 - e);
@@ -160,10 +191,12 @@ __device__ __forceinline__ double2 stencil_exact_c(const double2 *x, double e, c
         double E = lg.E[j - 1];
         double F = j + e;
         double G = j + (1 - e);
+        double yF = rcp_nr2(F);
+        double yG = rcp_nr2(G);
         double2 up = x[1 + j];
         double2 down = x[-j];
-        sum_re += E * (up.x / F + down.x / G);
-        sum_im += E * (up.y / F + down.y / G);
+        sum_re += E * (div_rcp(up.x, F, yF) + div_rcp(down.x, G, yG));
+        sum_im += E * (div_rcp(up.y, F, yF) + div_rcp(down.y, G, yG));
     }
     double2 up = x[1];
     double2 down = x[0];
@@ -206,19 +239,21 @@ __device__ __forceinline__ double2 stencil_fast_c(const double2 *x, double e, co
 }
 
 template <bool EXACT, int H>
-__device__ __forceinline__ double stencil(const double *x, double e, const LagrangeView &lg)
+__device__ __forceinline__ double stencil(const double *x, double e, const LagrangeView &lg,
+                                          double y_deps)
 {
     if constexpr (EXACT)
-        return stencil_exact<H>(x, e, lg);
+        return stencil_exact<H>(x, e, lg, y_deps);
     else
         return stencil_fast<H>(x, e, lg);
 }
 
 template <bool EXACT, int H>
-__device__ __forceinline__ double2 stencil_c(const double2 *x, double e, const LagrangeView &lg)
+__device__ __forceinline__ double2 stencil_c(const double2 *x, double e, const LagrangeView &lg,
+                                             double y_deps)
 {
     if constexpr (EXACT)
-        return stencil_exact_c<H>(x, e, lg);
+        return stencil_exact_c<H>(x, e, lg, y_deps);
     else
         return stencil_fast_c<H>(x, e, lg);
 }

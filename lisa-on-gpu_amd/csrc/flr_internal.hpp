@@ -86,11 +86,25 @@ struct TdiParams {
     int *err;
 };
 
+// Device row (orbit-table column / y_gw row) of each link in the canonical order
+// 12, 23, 31, 13, 32, 21.
+struct RowMap {
+    int row[6];
+};
+
+// Stock TDI dispatch: generation 1 or 2 and whether all six light travel times of the orbit
+// table are bitwise equal (then units with equal chain lengths share one delay).
+struct StockTdi {
+    int generation;  // 0: not a stock table
+    bool equal_arm;
+    RowMap rows;
+};
+
 // Launchers (flr_kernels.hip).
 hipError_t launch_projection(const ProjParams &p, const OrbitView &orb, const LagrangeView &lg,
                              bool exact, hipStream_t stream);
 hipError_t launch_tdi(const TdiParams &p, const OrbitView &orb, const LagrangeView &lg, bool exact,
-                      hipStream_t stream);
+                      const StockTdi &stock, hipStream_t stream);
 hipError_t launch_orbit_eval(const OrbitView &orb, const double *t, int n, int row, int sc,
                              double *ltt_out, double *pos_out, hipStream_t stream);
 hipError_t launch_clear(int *word, hipStream_t stream);

@@ -10,6 +10,7 @@
 //   walks every TDI unit of every channel for its sample, summing units of a channel in index
 //   order in registers (no atomics), then stores X, Y, Z or A, E, T.
 #include "flr_device.hpp"
+#include "flr_tdi_stock.hpp"
 
 namespace flr {
 
@@ -98,6 +99,7 @@ __global__ __launch_bounds__(kBlock) void projection_kernel(ProjParams p, OrbitV
     }
     __syncthreads();
     if (!valid) return;
+    const double y_deps = EXACT ? rcp_nr2(lg.deps) : 0.0;
 
     bool bad = false;
 #pragma unroll
@@ -105,14 +107,14 @@ __global__ __launch_bounds__(kBlock) void projection_kernel(ProjParams p, OrbitV
         const int lr = d_rec[s] - win_lo;
         bad |= (lr - (h - 1) < 0) || (lr + h >= win_len);
         double2 hr = make_double2(0.0, 0.0);
-        if (!bad) hr = stencil_c<EXACT, H>(hwin + lr, e_rec[s], lg);
+        if (!bad) hr = stencil_c<EXACT, H>(hwin + lr, e_rec[s], lg, y_deps);
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
             const int le = d_em[s][q] - win_lo;
             const bool bq = bad || (le - (h - 1) < 0) || (le + h >= win_len);
             double out = __builtin_nan("");
             if (!bq) {
-                const double2 he = stencil_c<EXACT, H>(hwin + le, e_em[s][q], lg);
+                const double2 he = stencil_c<EXACT, H>(hwin + le, e_em[s][q], lg, y_deps);
                 const double large = (he.x - hr.x) * xi_p[s][q] + (he.y - hr.y) * xi_c[s][q];
                 out = pre[s][q] * large;
             } else {
@@ -173,6 +175,7 @@ __global__ __launch_bounds__(kBlock) void tdi_kernel(TdiParams p, OrbitView orb,
     }
     __syncthreads();
     if (!valid) return;
+    const double y_deps = EXACT ? rcp_nr2(lg.deps) : 0.0;
 
     double acc[3];
 #pragma unroll
@@ -197,7 +200,7 @@ __global__ __launch_bounds__(kBlock) void tdi_kernel(TdiParams p, OrbitView orb,
             const int ld = d - win_lo;
             double val;
             if ((ld - (h - 1) >= 0) && (ld + h < win_len)) {
-                val = stencil<EXACT, H>(ywin + un.base_row * p.win_cap + ld, e, lg);
+                val = stencil<EXACT, H>(ywin + un.base_row * p.win_cap + ld, e, lg, y_deps);
             } else {
                 bad = true;
                 val = __builtin_nan("");
@@ -207,6 +210,85 @@ __global__ __launch_bounds__(kBlock) void tdi_kernel(TdiParams p, OrbitView orb,
         }
     }
     if (bad) atomicOr(p.err, kErrWindow);
+
+    if (p.aet) {
+        const double X = acc[0], Y = acc[1], Z = acc[2];
+        p.out[i] = (Z - X) / p.s2;
+        p.out[(size_t)p.num_delays + i] = (X - 2.0 * Y + Z) / p.s6;
+        p.out[(size_t)2 * p.num_delays + i] = (X + Y + Z) / p.s3;
+    } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) p.out[(size_t)c * p.num_delays + i] = acc[c];
+    }
+}
+
+// ------------------------------------------------------------------ equal-arm stock TDI
+// Same tiling and outputs as tdi_kernel, for the 1st/2nd-generation tables at order 25 when
+// all six light travel times of the orbit table are bitwise equal (equal-arm orbits): the
+// distinct delays t - m*L (m = chain length 0..3 or 0..7) are split once each and their
+// stencil reciprocals (exact form) or weights (fast form) are shared by the units of that
+// length -- 8 groups instead of 48 units for 2nd generation, with bit-identical results.
+
+template <int NX, bool EXACT>
+__global__ __launch_bounds__(kBlock) void tdi_equal_arm_kernel(TdiParams p, OrbitView orb,
+                                                           LagrangeView lg, RowMap rm)
+{
+    extern __shared__ double smem[];
+    double *ywin = smem;  // [6][win_cap]
+    const int tid = threadIdx.x;
+    const int i0 = p.grid_begin + (int)blockIdx.x * kBlock;
+    const int i = i0 + tid;
+    const int lo_valid = p.start;
+    const int hi_valid = p.num_delays - p.start;
+    const bool valid = (i >= lo_valid) && (i < hi_valid);
+    const bool in_range = (i >= 0) && (i < p.num_delays);
+    const int first = max(i0, lo_valid);
+    const int last = min(i0 + kBlock, hi_valid) - 1;
+
+    if (!valid && in_range && p.zero_edges) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) p.out[(size_t)c * p.num_delays + i] = 0.0;
+    }
+    if (first > last) return;
+
+    const int win_lo = first - p.halo_lo;
+    const int win_len = last + p.halo_hi - win_lo + 1;
+    if (win_len > p.win_cap || win_lo < 0 || win_lo + win_len > p.num_inputs) {
+        if (valid) {
+            atomicOr(p.err, kErrWindow);
+            for (int c = 0; c < 3; ++c) p.out[(size_t)c * p.num_delays + i] = __builtin_nan("");
+        }
+        return;
+    }
+    for (int r = 0; r < kNLinks; ++r) {
+        const double *src = p.y + (size_t)r * p.num_inputs + win_lo;
+        double *dst = ywin + r * p.win_cap;
+        for (int jj = tid; jj < win_len; jj += kBlock) dst[jj] = src[jj];
+    }
+
+    const int ic = valid ? i : first;
+    const double t = p.t_arr ? p.t_arr[ic] : p.t0 + (double)ic * p.dt;
+    const double t0_offset = p.t_arr ? p.t_arr[0] : p.t0 + (double)0 * p.dt;
+    double Lc[6];
+    {
+        const OrbitNode nd = orbit_node(orb, t);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) Lc[k] = orbit_val(nd, rm.row[k]);
+    }
+    int rowoff[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) rowoff[k] = rm.row[k] * p.win_cap - win_lo;
+    __syncthreads();
+    if (!valid) return;
+    const double y_deps = EXACT ? rcp_nr2(lg.deps) : 0.0;
+
+    double acc[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        acc[c] = (p.overwrite || p.aet) ? 0.0 : p.out[(size_t)c * p.num_delays + i];
+    if (!tdi_equal_arm<NX, EXACT>(t, Lc[0], t0_offset, p.fs, ywin, rowoff, win_lo, win_len, lg,
+                                  y_deps, acc))
+        atomicOr(p.err, kErrWindow);
 
     if (p.aet) {
         const double X = acc[0], Y = acc[1], Z = acc[2];
@@ -284,12 +366,33 @@ static hipError_t launch_tdi_t(const TdiParams &p, const OrbitView &orb, const L
     return hipGetLastError();
 }
 
+template <int NX, bool EXACT>
+static hipError_t launch_tdi_equal_t(const TdiParams &p, const OrbitView &orb,
+                                     const LagrangeView &lg, const RowMap &rm, int nblocks,
+                                     size_t shmem, hipStream_t stream)
+{
+    auto k = tdi_equal_arm_kernel<NX, EXACT>;
+    hipError_t err = set_lds_limit(k, shmem);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(k, dim3(nblocks), dim3(kBlock), shmem, stream, p, orb, lg, rm);
+    return hipGetLastError();
+}
+
 hipError_t launch_tdi(const TdiParams &p, const OrbitView &orb, const LagrangeView &lg, bool exact,
-                      hipStream_t stream)
+                      const StockTdi &stock_tdi, hipStream_t stream)
 {
     const int span = p.zero_edges ? p.num_delays : p.num_delays - 2 * p.start;
     if (span <= 0) return hipSuccess;
     const int nblocks = (span + kBlock - 1) / kBlock;
+    if (stock_tdi.generation != 0 && stock_tdi.equal_arm && lg.h == 13) {
+        const size_t shm = (size_t)kNLinks * p.win_cap * sizeof(double);
+        const RowMap &rm = stock_tdi.rows;
+        if (stock_tdi.generation == 1)
+            return exact ? launch_tdi_equal_t<8, true>(p, orb, lg, rm, nblocks, shm, stream)
+                         : launch_tdi_equal_t<8, false>(p, orb, lg, rm, nblocks, shm, stream);
+        return exact ? launch_tdi_equal_t<16, true>(p, orb, lg, rm, nblocks, shm, stream)
+                     : launch_tdi_equal_t<16, false>(p, orb, lg, rm, nblocks, shm, stream);
+    }
     const size_t shmem = ((size_t)kNLinks * p.win_cap + (size_t)kNLinks * kBlock) * sizeof(double);
     if (lg.h == 13)
         return exact ? launch_tdi_t<true, 13>(p, orb, lg, nblocks, shmem, stream)

@@ -6,7 +6,7 @@ Public API (mirrors the reference package `fastlisaresponse`):
     OrbitsWrap, LISAResponseWrap          -- backend.py (the binding-level classes)
     AET, get_overlap                      -- utility.py
 """
-from .orbits import EqualArmlengthOrbits, Orbits, StaticOrbits, YRSID_SI
+from .orbits import CartwheelOrbits, EqualArmlengthOrbits, Orbits, StaticOrbits, YRSID_SI
 from .utility import AET, get_overlap
 
 __version__ = "0.1.0"
@@ -23,5 +23,6 @@ def __getattr__(name):
     raise AttributeError(name)
 
 
-__all__ = ["pyResponseTDI", "ResponseWrapper", "Orbits", "EqualArmlengthOrbits", "StaticOrbits",
+__all__ = ["pyResponseTDI", "ResponseWrapper", "Orbits", "EqualArmlengthOrbits", "CartwheelOrbits",
+           "StaticOrbits",
            "OrbitsWrap", "LISAResponseWrap", "AET", "get_overlap", "YRSID_SI"]

@@ -179,6 +179,46 @@ class EqualArmlengthOrbits(Orbits):
         super().configure(t_arr=t_arr, dt=dt, linear_interp_setup=linear_interp_setup)
 
 
+class CartwheelOrbits(Orbits):
+    """The analytic cartwheel of EqualArmlengthOrbits with unequal, time-varying light travel
+    times: L_ij(t) = |x_i(t) - x_j(t - L_ij)| / c (receiver i, emitter j), solved by fixed-point
+    iteration, so L_ij != L_ji and all six differ.
+
+    Args:
+        armlength: nominal arm length (m) of the constellation.
+        t_max: span of the tables (s); 10 years by default.
+        dt_base: spacing of the base tables (s); 1e4 s by default.
+    """
+
+    def __init__(self, armlength: float = 2.5e9, t_max: float = 10.0 * YRSID_SI,
+                 dt_base: float = 1.0e4):
+        t_base = np.arange(0.0, t_max + 0.5 * dt_base, dt_base)
+        self.armlength = float(armlength)
+        x, ltt = self._model(t_base)
+        super().__init__(t_base, x, ltt, None, armlength)
+        self._dt_base = float(dt_base)
+
+    def _model(self, t: np.ndarray):
+        x = cartwheel_positions(t, self.armlength)
+        ltt = np.zeros((len(t), 6))
+        for r, link in enumerate(self.LINKS):
+            i, j = divmod(link, 10)
+            L = np.linalg.norm(x[:, i - 1] - x[:, j - 1], axis=-1) / C_SI
+            for _ in range(3):
+                xe = cartwheel_positions(t - L, self.armlength)[:, j - 1]
+                L = np.linalg.norm(x[:, i - 1] - xe, axis=-1) / C_SI
+            ltt[:, r] = L
+        return x, ltt
+
+    def _evaluate(self, t: np.ndarray):
+        return self._model(np.asarray(t, dtype=np.float64))
+
+    def configure(self, t_arr=None, dt=None, linear_interp_setup=False) -> None:
+        if t_arr is None and dt is None:
+            dt = self._dt_base
+        super().configure(t_arr=t_arr, dt=dt, linear_interp_setup=linear_interp_setup)
+
+
 class StaticOrbits(Orbits):
     """A frozen constellation (constant positions and light travel times) tabulated every
     `dt_base` seconds; useful for closed-form checks of the response."""

@@ -11,8 +11,11 @@ returned are torch tensors on the GPU. Differences, all deliberate:
 * For tdi_chan "AET"/"AE" the XYZ -> AET transform is fused into the TDI kernel; the `XYZ`
   property still returns X, Y, Z (recomputed on first access).
 * `get_tdi_delays(y_gw=...)` works (the reference's check references a missing attribute).
-* `exact=True` selects the reference stencil operation order (bitwise equal to the FP64 CPU
-  path); the default fast form agrees with it to ~1e-15 relative.
+* `exact=True` (the default) evaluates every stencil in the reference's operation order with
+  correctly rounded divisions: outputs are bitwise equal to the FP64 CPU path. `exact=False`
+  selects the reciprocal-free fast form, equal to ~1e-15 relative on A and E; the null channel
+  T (thousands of times smaller than X, Y, Z for low-frequency sources) then differs at the
+  1e-12..1e-11 relative level, since ulp-level differences in X, Y, Z do not cancel in it.
 """
 from __future__ import annotations
 
@@ -64,12 +67,13 @@ class pyResponseTDI:
         tdi_orbits (Orbits): stored for interface parity; the kernels use `orbits`.
         tdi_chan (str): "XYZ", "AET" or "AE".
         force_backend: None or a GPU backend name.
-        exact (bool): reference stencil operation order (bitwise FP64 CPU parity).
+        exact (bool): reference stencil operation order, bitwise FP64 CPU parity (default);
+            False selects the faster reciprocal-free stencils.
     """
 
     def __init__(self, sampling_frequency, num_pts, order=25, tdi="1st generation",
                  orbits: Optional[Orbits] = None, tdi_orbits: Optional[Orbits] = None,
-                 tdi_chan="XYZ", force_backend=None, exact: bool = False):
+                 tdi_chan="XYZ", force_backend=None, exact: bool = True):
         _check_backend(force_backend)
         self.sampling_frequency = sampling_frequency
         self.dt = 1 / sampling_frequency
@@ -81,7 +85,7 @@ class pyResponseTDI:
         self.tdi = tdi
         self.tdi_chan = tdi_chan
         self.exact = bool(exact)
-        self.device = device()
+        self._cpp_response = None
 
         self._fill_A_E()
         self.response_orbits = orbits
@@ -96,6 +100,8 @@ class pyResponseTDI:
 
         self._init_TDI_delays()
 
+        # host-side configuration is validated above; from here on the GPU is required
+        self.device = device()
         self.cpp_orbits = OrbitsWrap(*self.response_orbits.pycppdetector_args)
         self.cpp_response = LISAResponseWrap(self.cpp_orbits)
         self.cpp_response.set_lagrange(self.order, self.A_in, self.deps, self.num_A, self.E_in)

@@ -32,6 +32,18 @@
 #define ORACLE_NLINKS 6
 #define ORACLE_C_INV 3.3356409519815204e-09 /* LISAResponse.hh:7 */
 
+/* Mutation hooks for the tests that show the oracle's checks have teeth: a build with
+ * -DORACLE_MUTANT=k plants one plausible mistake (never set in the normal build):
+ *   1 receiver delay with the wrong sign (t + k.x/c)
+ *   2 the last link of every TDI delay chain dropped
+ *   3 cross polarisation projection with the wrong sign
+ *   4 receiver and emitter spacecraft swapped in the projection geometry
+ *   5 A channel with the wrong sign
+ *   6 stencil taps shifted by one sample */
+#ifndef ORACLE_MUTANT
+#define ORACLE_MUTANT 0
+#endif
+
 typedef struct {
     double dt;          /* orbit grid spacing (s) */
     int N;              /* number of orbit nodes */
@@ -91,6 +103,9 @@ void oracle_pos(const oracle_orbits *o, double t, int sc, double out[3])
 static double interp_single(const double *input, int h, int d, double e, const double *A_arr,
                             double deps, const double *E_arr)
 {
+#if ORACLE_MUTANT == 6
+    d += 1;
+#endif
     int ind = (int)(e / deps);
     double frac = (e - ind * deps) / deps;
     double A = A_arr[ind] * (1. - frac) + A_arr[ind + 1] * frac;
@@ -205,8 +220,13 @@ int oracle_get_response(double *y_gw, const double *t_data, const double *k, con
             double L = oracle_ltt(orbits, t, link);
             double t_rec = t;
             double t_em = t - L;
+#if ORACLE_MUTANT == 4
+            oracle_pos(orbits, t_rec, sc_e, x_rec);
+            oracle_pos(orbits, t_em, sc_r, x_em);
+#else
             oracle_pos(orbits, t_rec, sc_r, x_rec);
             oracle_pos(orbits, t_em, sc_e, x_em);
+#endif
             double norm = 0.0;
             for (int c = 0; c < 3; c++) {
                 double nt = x_rec[c] - x_em[c];
@@ -220,12 +240,19 @@ int oracle_get_response(double *y_gw, const double *t_data, const double *k, con
             double v_dot_n = dot3(v, n);
             double xi_p = 0.5 * ((u_dot_n * u_dot_n) - (v_dot_n * v_dot_n));
             double xi_c = u_dot_n * v_dot_n;
+#if ORACLE_MUTANT == 3
+            xi_c = -xi_c;
+#endif
 
             double k_dot_n = dot3(k, n);
             double k_dot_x_rec = dot3(k, x_rec);
             double k_dot_x_em = dot3(k, x_em);
 
+#if ORACLE_MUTANT == 1
+            double delay_rec = t_rec + k_dot_x_rec * ORACLE_C_INV;
+#else
             double delay_rec = t_rec - k_dot_x_rec * ORACLE_C_INV;
+#endif
             double delay_em = t_em - k_dot_x_em * ORACLE_C_INV;
 
             int d_rec, d_em;
@@ -286,6 +313,9 @@ int oracle_get_tdi_delays(double *delayed_links, const double *input_links, int 
             int channel = channels[unit_i];
             double sign = tdi_signs_in[unit_i];
             double delay = t;
+#if ORACLE_MUTANT == 2
+            if (unit_length > 1) unit_length -= 1;
+#endif
             for (int sub_i = 0; sub_i < unit_length; sub_i++) {
                 int combination_link = tdi_link_combinations[unit_start + sub_i];
                 if (combination_link != -11) delay -= oracle_ltt(orbits, t, combination_link);
@@ -312,7 +342,11 @@ void oracle_aet(const double *X, const double *Y, const double *Z, double *A, do
     const double s2 = sqrt(2.0), s6 = sqrt(6.0), s3 = sqrt(3.0);
     for (int64_t i = 0; i < n; i++) {
         double x = X[i], y = Y[i], z = Z[i];
+#if ORACLE_MUTANT == 5
+        A[i] = (x - z) / s2;
+#else
         A[i] = (z - x) / s2;
+#endif
         E[i] = (x - 2.0 * y + z) / s6;
         T[i] = (x + y + z) / s3;
     }

@@ -72,35 +72,50 @@ class _Orbits(ctypes.Structure):
 _lib = None
 
 
+def build_mutant(mutant: int, out_dir: str) -> str:
+    """Compile the oracle with one planted mistake (-DORACLE_MUTANT=mutant) into out_dir; used
+    by the tests that show the oracle's checks detect such mistakes."""
+    out = os.path.join(out_dir, f"liboracle_mutant{mutant}.so")
+    subprocess.run(["gcc", "-O2", "-std=c11", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
+                    "-fopenmp", "-shared", f"-DORACLE_MUTANT={mutant}", "-o", out,
+                    os.path.join(ORACLE_DIR, "flr_oracle.c"), "-lm"], check=True)
+    return out
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = ctypes.CDLL(LIB_PATH)
-        vp, ci, cd = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
-        L.oracle_ltt.argtypes = [vp, cd, ci]
-        L.oracle_ltt.restype = cd
-        L.oracle_pos.argtypes = [vp, cd, ci, vp]
-        L.oracle_pos.restype = None
-        L.oracle_link_row.argtypes = [vp, ci]
-        L.oracle_interp_single.argtypes = [vp, ci, ci, cd, vp, cd, vp]
-        L.oracle_interp_single.restype = cd
-        L.oracle_interp_cplx.argtypes = [vp, vp, ci, ci, cd, vp, cd, vp]
-        L.oracle_interp_cplx.restype = None
-        L.oracle_split_delay.argtypes = [cd, cd, cd, vp, vp]
-        L.oracle_split_delay.restype = None
-        L.oracle_get_response.argtypes = [vp, vp, vp, vp, vp, cd, ci, vp, ci, ci, cd, ci, vp, cd,
-                                          ci, vp, ci, vp, ci]
-        L.oracle_get_response.restype = ci
-        L.oracle_get_tdi_delays.argtypes = [vp, vp, ci, ci, vp, vp, vp, vp, vp, vp, vp, ci, ci, ci,
-                                            cd, ci, vp, cd, ci, vp, ci, vp, ci]
-        L.oracle_get_tdi_delays.restype = ci
-        L.oracle_aet.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.c_int64]
-        L.oracle_aet.restype = None
-        L.oracle_max_threads.restype = ci
-        _lib = L
+        _lib = load(LIB_PATH)
     return _lib
+
+
+def load(path: str):
+    """Load an oracle library and declare its C signatures."""
+    L = ctypes.CDLL(path)
+    vp, ci, cd = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    L.oracle_ltt.argtypes = [vp, cd, ci]
+    L.oracle_ltt.restype = cd
+    L.oracle_pos.argtypes = [vp, cd, ci, vp]
+    L.oracle_pos.restype = None
+    L.oracle_link_row.argtypes = [vp, ci]
+    L.oracle_interp_single.argtypes = [vp, ci, ci, cd, vp, cd, vp]
+    L.oracle_interp_single.restype = cd
+    L.oracle_interp_cplx.argtypes = [vp, vp, ci, ci, cd, vp, cd, vp]
+    L.oracle_interp_cplx.restype = None
+    L.oracle_split_delay.argtypes = [cd, cd, cd, vp, vp]
+    L.oracle_split_delay.restype = None
+    L.oracle_get_response.argtypes = [vp, vp, vp, vp, vp, cd, ci, vp, ci, ci, cd, ci, vp, cd,
+                                      ci, vp, ci, vp, ci]
+    L.oracle_get_response.restype = ci
+    L.oracle_get_tdi_delays.argtypes = [vp, vp, ci, ci, vp, vp, vp, vp, vp, vp, vp, ci, ci, ci,
+                                        cd, ci, vp, cd, ci, vp, ci, vp, ci]
+    L.oracle_get_tdi_delays.restype = ci
+    L.oracle_aet.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.c_int64]
+    L.oracle_aet.restype = None
+    L.oracle_max_threads.restype = ci
+    return L
 
 
 def _p(a: np.ndarray) -> int:

@@ -1,0 +1,49 @@
+"""The C-ABI library loads and exports every entry point include/flr.h declares (CPU only:
+no compute calls, no device needed)."""
+import ctypes
+import os
+import re
+
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "flr.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*[\w\s\*]*?\b(flr_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_header_declares_the_boundary():
+    names = declared_functions()
+    for must in ("flr_orbits_create", "flr_response_create", "flr_get_response",
+                 "flr_get_tdi_delays", "flr_projection", "flr_tdi", "flr_last_error"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    from fastlisaresponse_amd import _capi
+    lib = ctypes.CDLL(_capi.library_path())
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_the_header():
+    from fastlisaresponse_amd import _capi
+    assert sorted(_capi.SIGNATURES) == declared_functions()
+
+
+def test_host_only_entry_points():
+    from fastlisaresponse_amd import _capi
+    lib = _capi.lib()
+    assert lib.flr_version().decode().startswith("lisa-on-gpu_amd")
+    assert lib.flr_last_error().decode() == ""
+    # argument validation happens before any device work
+    out = ctypes.c_void_p()
+    assert lib.flr_response_create(None, ctypes.byref(out)) == _capi.FLR_ERR_NO_ORBITS
+    assert "orbits" in lib.flr_last_error().decode()
+    assert lib.flr_orbits_create(0.0, 5, None, None, None, None, None, None, 2.5e9,
+                                 ctypes.byref(out)) == _capi.FLR_ERR_INVALID_ARGUMENT
+    assert lib.flr_projection(None, None, None, 0, None, 0.0, 1.0, 1.0, None, None, None, 0, 0,
+                              None) == _capi.FLR_ERR_INVALID_ARGUMENT

@@ -1,0 +1,103 @@
+"""Host-side behaviour of the Python layer that needs no GPU: argument validation (errors the
+reference raises before any kernel runs), orbit providers and their table layout."""
+import numpy as np
+import pytest
+
+from conftest import YRSID_SI
+
+C_SI = 299792458.0
+
+
+def test_cpu_backend_is_refused():
+    from fastlisaresponse_amd.response import ResponseWrapper, pyResponseTDI
+    with pytest.raises(ValueError, match="only the GPU"):
+        pyResponseTDI(0.1, 1000, force_backend="cpu")
+    with pytest.raises(ValueError, match="only the GPU"):
+        ResponseWrapper(lambda *a, **k: None, 0.1, 10.0, 6, 7, force_backend="cpu")
+
+
+def test_bad_tdi_raises_before_gpu():
+    from fastlisaresponse_amd.orbits import EqualArmlengthOrbits
+    from fastlisaresponse_amd.response import pyResponseTDI
+    o = EqualArmlengthOrbits(t_max=1e7)
+    with pytest.raises(ValueError, match="tdi kwarg"):
+        pyResponseTDI(0.1, 1000, tdi="3rd generation", orbits=o)
+
+
+def test_n_overide_must_be_int():
+    from fastlisaresponse_amd.orbits import EqualArmlengthOrbits
+    from fastlisaresponse_amd.response import ResponseWrapper
+    with pytest.raises(ValueError, match="n_overide"):
+        ResponseWrapper(lambda *a, **k: None, 0.1, 10.0, 6, 7, n_overide=10.5,
+                        orbits=EqualArmlengthOrbits(t_max=1e7))
+
+
+def test_tobs_is_clamped_to_the_orbits():
+    from fastlisaresponse_amd.orbits import EqualArmlengthOrbits
+    from fastlisaresponse_amd.response import ResponseWrapper
+    o = EqualArmlengthOrbits(t_max=1e7)
+    with pytest.warns(UserWarning, match="Tobs is larger"):
+        with pytest.raises(Exception):  # the warning precedes the GPU-only construction
+            ResponseWrapper(lambda *a, **k: None, 5.0, 10.0, 6, 7, orbits=o, tdi="bogus")
+
+
+def test_pycppdetector_args_layout():
+    from fastlisaresponse_amd.orbits import EqualArmlengthOrbits
+    o = EqualArmlengthOrbits(t_max=2e6, dt_base=1e4)
+    with pytest.raises(ValueError):
+        o.pycppdetector_args
+    o.configure(linear_interp_setup=True)
+    dt, N, n_arr, ltt, x, links, sc_r, sc_e, arm = o.pycppdetector_args
+    assert dt == 1e4 and N == len(o.t) == 201 and o.t[0] == 0.0
+    assert n_arr.shape == (N * 18,) and ltt.shape == (N * 6,) and x.shape == (N * 9,)
+    assert list(links) == [12, 23, 31, 13, 32, 21]
+    assert list(sc_r) == [1, 2, 3, 1, 3, 2] and list(sc_e) == [2, 3, 1, 3, 2, 1]
+    assert links.dtype == np.int32 and arm == 2.5e9
+    # rows are time-major: ltt[t][link], x[t][sc][xyz], n[t][link][xyz]
+    assert np.array_equal(ltt.reshape(N, 6), o.ltt)
+    assert np.array_equal(x.reshape(N, 3, 3), o.x)
+    assert np.allclose(np.linalg.norm(n_arr.reshape(N, 6, 3), axis=-1), 1.0)
+
+
+def test_equal_arm_cartwheel_geometry():
+    from fastlisaresponse_amd.orbits import AU_SI, EqualArmlengthOrbits
+    o = EqualArmlengthOrbits(t_max=YRSID_SI)
+    o.configure(linear_interp_setup=True)
+    assert np.all(o.ltt == 2.5e9 / C_SI)
+    for i, j in ((0, 1), (1, 2), (2, 0)):
+        arm = np.linalg.norm(o.x[:, i] - o.x[:, j], axis=-1)
+        assert np.allclose(arm, 2.5e9, rtol=1e-2)
+    centre = o.x.mean(axis=1)
+    assert np.allclose(np.linalg.norm(centre, axis=-1), AU_SI, rtol=1e-4)
+    # one orbit per year: the guiding centre returns to its start
+    assert np.allclose(centre[0], centre[np.argmin(np.abs(o.t - YRSID_SI))], atol=1e-3 * AU_SI)
+
+
+def test_unequal_arm_cartwheel_light_times():
+    from fastlisaresponse_amd.orbits import CartwheelOrbits, cartwheel_positions
+    o = CartwheelOrbits(t_max=3e6)
+    o.configure(linear_interp_setup=True)
+    assert len(np.unique(o.ltt[50])) == 6
+    for r, link in enumerate(o.LINKS):
+        i, j = divmod(link, 10)
+        L = o.ltt[:, r]
+        xe = cartwheel_positions(o.t - L)[:, j - 1]
+        resid = np.linalg.norm(o.x[:, i - 1] - xe, axis=-1) / C_SI - L
+        assert np.abs(resid).max() < 1e-9
+
+
+def test_static_orbits_are_constant():
+    from fastlisaresponse_amd.orbits import StaticOrbits
+    pos = np.arange(9.0).reshape(3, 3) * 1e9
+    o = StaticOrbits(pos, np.arange(6) + 8.0, t_max=1e5)
+    o.configure(linear_interp_setup=True)
+    assert np.array_equal(o.x, np.broadcast_to(pos, o.x.shape))
+    assert np.array_equal(o.ltt[7], np.arange(6) + 8.0)
+
+
+def test_overlap_helper():
+    from fastlisaresponse_amd.utility import get_overlap
+    rng = np.random.default_rng(0)
+    a = [rng.normal(size=256) for _ in range(3)]
+    assert abs(get_overlap(a, a) - 1.0) < 1e-12
+    assert abs(get_overlap(a, [-x for x in a]) + 1.0) < 1e-12
