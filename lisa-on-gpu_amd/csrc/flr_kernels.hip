@@ -49,24 +49,31 @@ __global__ __launch_bounds__(kBlock) void projection_kernel(ProjParams p, OrbitV
     }
     for (int jj = tid; jj < win_len; jj += kBlock) hwin[jj] = p.h[win_lo + jj];
 
-    // Geometry of this lane's sample (overlaps the window loads in flight).
     const int ic = valid ? i : first;
     const double t = p.t_data ? p.t_data[ic] : p.t0 + (double)ic * p.dt;
     const double t0_offset = p.t_data ? p.t_data[0] : p.t0 + (double)0 * p.dt;
-
-    int d_rec[3];
-    double e_rec[3];
-    int d_em[3][2];
-    double e_em[3][2], xi_p[3][2], xi_c[3][2], pre[3][2];
-
     const OrbitNode nr = orbit_node(orb, t);
+    __syncthreads();
+    if (!valid) return;
+    const double y_deps = EXACT ? rcp_nr2(lg.deps) : 0.0;
+
+    // Per receiving spacecraft: its stencil (shared by its two links), then each link's
+    // emitter geometry and stencil; the window is sized from the orbit extent, so nothing has
+    // to be known about the delays before it is staged. Operation order per link follows
+    // LISAResponse.cu:559-668.
+    bool bad = false;
 #pragma unroll
     for (int s = 0; s < 3; ++s) {
         double xr[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) xr[c] = orbit_val(nr, 6 + 3 * s + c);
-        const double kxr = dot3(p.k, xr);
-        split_delay(t - kxr * kCInv, t0_offset, p.fs, d_rec[s], e_rec[s]);
+        int d_rec;
+        double e_rec;
+        split_delay(t - dot3(p.k, xr) * kCInv, t0_offset, p.fs, d_rec, e_rec);
+        const int lr = d_rec - win_lo;
+        const bool br = (lr - (h - 1) < 0) || (lr + h >= win_len);
+        double2 hr = make_double2(0.0, 0.0);
+        if (!br) hr = stencil_c<EXACT, H>(hwin + lr, e_rec, lg, y_deps);
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
             const int row = p.rec_row[s][q];
@@ -89,40 +96,26 @@ __global__ __launch_bounds__(kBlock) void projection_kernel(ProjParams p, OrbitV
             for (int c = 0; c < 3; ++c) n[c] = n[c] / norm;
             const double un = dot3(p.u, n);
             const double vn = dot3(p.v, n);
-            xi_p[s][q] = 0.5 * ((un * un) - (vn * vn));
-            xi_c[s][q] = un * vn;
+            const double xi_p = 0.5 * ((un * un) - (vn * vn));
+            const double xi_c = un * vn;
             const double kn = dot3(p.k, n);
-            const double kxe = dot3(p.k, xe);
-            pre[s][q] = 1. / (1. - kn);
-            split_delay(t_em - kxe * kCInv, t0_offset, p.fs, d_em[s][q], e_em[s][q]);
-        }
-    }
-    __syncthreads();
-    if (!valid) return;
-    const double y_deps = EXACT ? rcp_nr2(lg.deps) : 0.0;
-
-    bool bad = false;
-#pragma unroll
-    for (int s = 0; s < 3; ++s) {
-        const int lr = d_rec[s] - win_lo;
-        bad |= (lr - (h - 1) < 0) || (lr + h >= win_len);
-        double2 hr = make_double2(0.0, 0.0);
-        if (!bad) hr = stencil_c<EXACT, H>(hwin + lr, e_rec[s], lg, y_deps);
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int le = d_em[s][q] - win_lo;
-            const bool bq = bad || (le - (h - 1) < 0) || (le + h >= win_len);
+            int d_em;
+            double e_em;
+            split_delay(t_em - dot3(p.k, xe) * kCInv, t0_offset, p.fs, d_em, e_em);
+            const int le = d_em - win_lo;
+            const bool bq = br || (le - (h - 1) < 0) || (le + h >= win_len);
             double out = __builtin_nan("");
             if (!bq) {
-                const double2 he = stencil_c<EXACT, H>(hwin + le, e_em[s][q], lg, y_deps);
-                const double large = (he.x - hr.x) * xi_p[s][q] + (he.y - hr.y) * xi_c[s][q];
-                out = pre[s][q] * large;
-            } else {
-                atomicOr(p.err, kErrWindow);
+                const double2 he = stencil_c<EXACT, H>(hwin + le, e_em, lg, y_deps);
+                const double pre = 1. / (1. - kn);
+                const double large = (he.x - hr.x) * xi_p + (he.y - hr.y) * xi_c;
+                out = pre * large;
             }
-            p.y[(size_t)p.rec_row[s][q] * p.num_delays + i] = out;
+            bad |= bq;
+            p.y[(size_t)row * p.num_delays + i] = out;
         }
     }
+    if (bad) atomicOr(p.err, kErrWindow);
 }
 
 // ------------------------------------------------------------------ TDI
