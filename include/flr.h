@@ -151,6 +151,37 @@ int flr_tdi(flr_response *response, double *out, const double *y_gw, int num_pts
 /* Synchronise `stream`, then read and clear the handle's latched device error word. */
 int flr_check_errors(flr_response *response, void *stream);
 
+/* ------------------------------------------------------------------ batches
+ * No counterpart in the reference (one source per call there): many sources of uneven length
+ * run as one persistent projection launch and one persistent TDI launch over a flattened
+ * (source, tile) list. Sources are packed back to back: source s occupies samples
+ * [offset, offset + num_pts) of h_all (complex128), rows [6*offset, 6*(offset + num_pts)) of
+ * y_all laid out [6][num_pts], and [3*offset, 3*(offset + num_pts)) of out_all ([3][num_pts]).
+ * Sample times are t0 + i*dt per source. */
+typedef struct {
+    int64_t offset;                /* first sample of the source in the packed arrays */
+    int32_t num_pts;               /* samples of the source */
+    int32_t projections_start_ind; /* as pyResponseTDI.get_projections computes it */
+    int32_t tdi_start_ind;
+    int32_t reserved;
+    double t0;                     /* time of the source's first sample (s) */
+    double k[3], u[3], v[3];       /* sky propagation and polarisation basis */
+} flr_source;
+
+typedef struct flr_batch flr_batch;
+
+/* Validate the sources, upload their tables and bind the packed device arrays. `flags` is fixed
+ * for the batch (FLR_FLAG_EXACT, FLR_FLAG_ZERO_EDGES, FLR_FLAG_AET, FLR_FLAG_OVERWRITE).
+ * Allocates device memory once here; flr_batch_run never allocates (graph-capturable). */
+int flr_batch_create(flr_response *response, const flr_source *sources, int num_sources,
+                     const double *h_all, double *y_all, double *out_all, double dt,
+                     double sampling_frequency, int flags, flr_batch **out);
+/* Enqueue the projection and the TDI launches of the batch on `stream`. */
+int flr_batch_run(flr_batch *batch, void *stream);
+/* Total samples of the batch. */
+int64_t flr_batch_samples(const flr_batch *batch);
+void flr_batch_destroy(flr_batch *batch);
+
 /* Stencil form used by the drop-in entry points flr_get_response / flr_get_tdi_delays:
  * nonzero (default) the reference operation order, bitwise equal to the FP64 CPU path;
  * 0 the faster reciprocal-free form (equal to ~1e-15 relative per stencil). */

@@ -133,6 +133,19 @@ struct flr_response {
     }
 };
 
+struct flr_batch {
+    flr_response *r = nullptr;
+    int flags = 0;
+    double dt = 0.0, fs = 0.0;
+    int num_sources = 0;
+    int64_t samples = 0;
+    ProjSource *d_proj = nullptr;
+    TdiSource *d_tdi = nullptr;
+    int *d_proj_tiles = nullptr;  // [num_sources + 1] prefix sums
+    int *d_tdi_tiles = nullptr;
+    int proj_tiles = 0, tdi_tiles = 0;
+};
+
 namespace {
 
 int set_lagrange_impl(flr_response *r, int order, const double *A_in, double deps, int num_A,
@@ -299,6 +312,70 @@ int set_tdi_impl(flr_response *r, const int32_t *unit_starts, const int32_t *uni
 
 constexpr size_t kMaxLds = 160 * 1024;
 
+// Launch configuration shared by all sources of a projection launch: receiver/emitter rows,
+// and halos sized from the orbit extent (max |x|/c and max light travel time).
+int make_proj_config(const flr_response *r, double dt, double fs, int flags, ProjConfig &cfg)
+{
+    const flr_orbits *o = r->orbits;
+    const int h = (r->order + 1) / 2;
+    cfg = ProjConfig{};
+    cfg.dt = dt;
+    cfg.fs = fs;
+    cfg.zero_edges = (flags & FLR_FLAG_ZERO_EDGES) ? 1 : 0;
+    for (int s = 0; s < 3; ++s) {
+        int q = 0;
+        for (int row = 0; row < kNLinks; ++row) {
+            if (o->sc_r[row] != s + 1) continue;
+            if (q == 2) return fail(FLR_ERR_INVALID_ARGUMENT, "a spacecraft receives more than two links");
+            cfg.rec_row[s][q] = row;
+            cfg.em_sc[s][q] = o->sc_e[row] - 1;
+            ++q;
+        }
+        if (q != 2) return fail(FLR_ERR_INVALID_ARGUMENT, "every spacecraft must receive two links");
+    }
+    const double rc = o->max_radius * kCInv;
+    cfg.halo_lo = (int)std::ceil((o->max_ltt + rc) * fs) + h + 4;
+    cfg.halo_hi = (int)std::ceil(rc * fs) + h + 4;
+    cfg.win_cap = kBlock + cfg.halo_lo + cfg.halo_hi;
+    if ((size_t)cfg.win_cap * sizeof(double2) > kMaxLds)
+        return fail(FLR_ERR_UNSUPPORTED, "projection delay window exceeds LDS (sampling too fine)");
+    cfg.err = r->d_err;
+    return FLR_OK;
+}
+
+// Launch configuration of a TDI launch: the resident unit table, halos from the longest delay
+// chain, output mode, and which kernel family applies.
+int make_tdi_config(const flr_response *r, double dt, double fs, int flags, TdiConfig &cfg,
+                    StockTdi &stock)
+{
+    const flr_orbits *o = r->orbits;
+    const int h = (r->order + 1) / 2;
+    cfg = TdiConfig{};
+    cfg.dt = dt;
+    cfg.fs = fs;
+    cfg.units = r->d_units;
+    cfg.chain_rows = r->d_chain;
+    for (int c = 0; c < 4; ++c) cfg.ch_begin[c] = r->ch_begin[c];
+    cfg.halo_lo = (int)std::ceil(r->max_chain * o->max_ltt * fs) + h + 4;
+    cfg.halo_hi = h + 4;
+    cfg.win_cap = kBlock + cfg.halo_lo + cfg.halo_hi;
+    if (((size_t)kNLinks * cfg.win_cap + (size_t)kNLinks * kBlock) * sizeof(double) > kMaxLds)
+        return fail(FLR_ERR_UNSUPPORTED, "TDI delay window exceeds LDS (sampling too fine)");
+    cfg.zero_edges = (flags & FLR_FLAG_ZERO_EDGES) ? 1 : 0;
+    cfg.aet = (flags & FLR_FLAG_AET) ? 1 : 0;
+    cfg.overwrite = ((flags & FLR_FLAG_OVERWRITE) || cfg.aet) ? 1 : 0;
+    cfg.s2 = std::sqrt(2.0);
+    cfg.s6 = std::sqrt(6.0);
+    cfg.s3 = std::sqrt(3.0);
+    cfg.err = r->d_err;
+    stock = StockTdi{};
+    stock.generation = r->stock_gen;
+    stock.equal_arm = o->equal_arm;
+    const int canon_links[6] = {12, 23, 31, 13, 32, 21};
+    for (int k = 0; k < 6; ++k) stock.rows.row[k] = o->row_of(canon_links[k]);
+    return FLR_OK;
+}
+
 int projection_impl(flr_response *r, double *y_gw, const double *input_in, int num_delays,
                     int num_inputs, const double *t_data, double t0, double dt, double fs,
                     const double *k, const double *u, const double *v, int start, int flags,
@@ -311,45 +388,24 @@ int projection_impl(flr_response *r, double *y_gw, const double *input_in, int n
     if (num_delays < 1 || num_inputs < 1) return fail(FLR_ERR_INVALID_ARGUMENT, "empty input");
     if (start < 0) return fail(FLR_ERR_INVALID_ARGUMENT, "projections_start_ind must be >= 0");
     if (!(fs > 0.0)) return fail(FLR_ERR_INVALID_ARGUMENT, "sampling_frequency must be > 0");
-    const flr_orbits *o = r->orbits;
-    const int h = (r->order + 1) / 2;
-
-    ProjParams p{};
-    p.y = y_gw;
-    p.h = reinterpret_cast<const double2 *>(input_in);
-    p.t_data = (flags & FLR_FLAG_T_UNIFORM) ? nullptr : t_data;
-    p.num_delays = num_delays;
-    p.num_inputs = num_inputs;
-    p.start = start;
-    p.zero_edges = (flags & FLR_FLAG_ZERO_EDGES) ? 1 : 0;
-    p.grid_begin = p.zero_edges ? 0 : start;
-    p.t0 = t0;
-    p.dt = dt;
-    p.fs = fs;
+    ProjConfig cfg;
+    int st = make_proj_config(r, dt, fs, flags, cfg);
+    if (st) return st;
+    ProjSource src{};
+    src.y = y_gw;
+    src.h = reinterpret_cast<const double2 *>(input_in);
+    src.t_data = (flags & FLR_FLAG_T_UNIFORM) ? nullptr : t_data;
+    src.num_delays = num_delays;
+    src.num_inputs = num_inputs;
+    src.start = start;
+    src.t0 = t0;
     for (int c = 0; c < 3; ++c) {
-        p.k[c] = k[c];
-        p.u[c] = u[c];
-        p.v[c] = v[c];
+        src.k[c] = k[c];
+        src.u[c] = u[c];
+        src.v[c] = v[c];
     }
-    for (int s = 0; s < 3; ++s) {
-        int q = 0;
-        for (int row = 0; row < kNLinks; ++row) {
-            if (o->sc_r[row] != s + 1) continue;
-            if (q == 2) return fail(FLR_ERR_INVALID_ARGUMENT, "a spacecraft receives more than two links");
-            p.rec_row[s][q] = row;
-            p.em_sc[s][q] = o->sc_e[row] - 1;
-            ++q;
-        }
-        if (q != 2) return fail(FLR_ERR_INVALID_ARGUMENT, "every spacecraft must receive two links");
-    }
-    const double rc = o->max_radius * kCInv;
-    p.halo_lo = (int)std::ceil((o->max_ltt + rc) * fs) + h + 4;
-    p.halo_hi = (int)std::ceil(rc * fs) + h + 4;
-    p.win_cap = kBlock + p.halo_lo + p.halo_hi;
-    if ((size_t)p.win_cap * sizeof(double2) > kMaxLds)
-        return fail(FLR_ERR_UNSUPPORTED, "projection delay window exceeds LDS (sampling too fine)");
-    p.err = r->d_err;
-    hipError_t e = launch_projection(p, o->view(), r->lagrange(), (flags & FLR_FLAG_EXACT) != 0, stream);
+    hipError_t e = launch_projection(cfg, src, r->orbits->view(), r->lagrange(),
+                                     (flags & FLR_FLAG_EXACT) != 0, stream);
     if (e != hipSuccess) return hip_fail(e, "projection launch");
     return FLR_OK;
 }
@@ -366,41 +422,20 @@ int tdi_impl(flr_response *r, double *out, const double *y_gw, int num_inputs, i
     if (num_delays < 1 || num_inputs < 1) return fail(FLR_ERR_INVALID_ARGUMENT, "empty input");
     if (start < 0) return fail(FLR_ERR_INVALID_ARGUMENT, "tdi_start_ind must be >= 0");
     if (!(fs > 0.0)) return fail(FLR_ERR_INVALID_ARGUMENT, "sampling_frequency must be > 0");
-    const flr_orbits *o = r->orbits;
-    const int h = (r->order + 1) / 2;
-
-    TdiParams p{};
-    p.out = out;
-    p.y = y_gw;
-    p.t_arr = (flags & FLR_FLAG_T_UNIFORM) ? nullptr : t_arr;
-    p.num_inputs = num_inputs;
-    p.num_delays = num_delays;
-    p.start = start;
-    p.zero_edges = (flags & FLR_FLAG_ZERO_EDGES) ? 1 : 0;
-    p.grid_begin = p.zero_edges ? 0 : start;
-    p.t0 = t0;
-    p.dt = dt;
-    p.fs = fs;
-    p.units = r->d_units;
-    p.chain_rows = r->d_chain;
-    for (int c = 0; c < 4; ++c) p.ch_begin[c] = r->ch_begin[c];
-    p.halo_lo = (int)std::ceil(r->max_chain * o->max_ltt * fs) + h + 4;
-    p.halo_hi = h + 4;
-    p.win_cap = kBlock + p.halo_lo + p.halo_hi;
-    if (((size_t)kNLinks * p.win_cap + (size_t)kNLinks * kBlock) * sizeof(double) > kMaxLds)
-        return fail(FLR_ERR_UNSUPPORTED, "TDI delay window exceeds LDS (sampling too fine)");
-    p.aet = (flags & FLR_FLAG_AET) ? 1 : 0;
-    p.overwrite = ((flags & FLR_FLAG_OVERWRITE) || p.aet) ? 1 : 0;
-    p.s2 = std::sqrt(2.0);
-    p.s6 = std::sqrt(6.0);
-    p.s3 = std::sqrt(3.0);
-    p.err = r->d_err;
-    StockTdi st{};
-    st.generation = r->stock_gen;
-    st.equal_arm = o->equal_arm;
-    const int canon_links[6] = {12, 23, 31, 13, 32, 21};
-    for (int k = 0; k < 6; ++k) st.rows.row[k] = o->row_of(canon_links[k]);
-    hipError_t e = launch_tdi(p, o->view(), r->lagrange(), (flags & FLR_FLAG_EXACT) != 0, st, stream);
+    TdiConfig cfg;
+    StockTdi stock;
+    int st = make_tdi_config(r, dt, fs, flags, cfg, stock);
+    if (st) return st;
+    TdiSource src{};
+    src.out = out;
+    src.y = y_gw;
+    src.t_arr = (flags & FLR_FLAG_T_UNIFORM) ? nullptr : t_arr;
+    src.num_inputs = num_inputs;
+    src.num_delays = num_delays;
+    src.start = start;
+    src.t0 = t0;
+    hipError_t e = launch_tdi(cfg, src, r->orbits->view(), r->lagrange(),
+                              (flags & FLR_FLAG_EXACT) != 0, stock, stream);
     if (e != hipSuccess) return hip_fail(e, "tdi launch");
     return FLR_OK;
 }
@@ -694,6 +729,115 @@ int flr_get_tdi_delays(flr_response *r, double *delayed_links, const double *inp
     if (st) return st;
     return check_errors_impl(r, s);
 }
+
+void flr_batch_destroy(flr_batch *b)
+{
+    if (!b) return;
+    DeviceGuard g(b->r ? b->r->device : 0);
+    if (b->d_proj) (void)hipFree(b->d_proj);
+    if (b->d_tdi) (void)hipFree(b->d_tdi);
+    if (b->d_proj_tiles) (void)hipFree(b->d_proj_tiles);
+    if (b->d_tdi_tiles) (void)hipFree(b->d_tdi_tiles);
+    delete b;
+}
+
+int flr_batch_create(flr_response *r, const flr_source *sources, int num_sources,
+                     const double *h_all, double *y_all, double *out_all, double dt,
+                     double sampling_frequency, int flags, flr_batch **out)
+{
+    if (!out) return fail(FLR_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    if (!r || !r->orbits) return fail(FLR_ERR_NO_ORBITS, "Must add orbits with add_orbit_information method.");
+    if (!r->lag_set || !r->tdi_set) return fail(FLR_ERR_INVALID_ARGUMENT, "Lagrange and TDI tables must be set");
+    if (num_sources < 1 || !sources) return fail(FLR_ERR_INVALID_ARGUMENT, "no sources");
+    if (!h_all || !y_all || !out_all) return fail(FLR_ERR_INVALID_ARGUMENT, "NULL packed array");
+    if (!(sampling_frequency > 0.0) || !(dt > 0.0)) return fail(FLR_ERR_INVALID_ARGUMENT, "dt and sampling_frequency must be > 0");
+    DeviceGuard g(r->device);
+    const int zero_edges = (flags & FLR_FLAG_ZERO_EDGES) ? 1 : 0;
+    std::vector<ProjSource> ps(num_sources);
+    std::vector<TdiSource> ts(num_sources);
+    std::vector<int> pt(num_sources + 1, 0), tt(num_sources + 1, 0);
+    int64_t samples = 0;
+    const double2 *h2 = reinterpret_cast<const double2 *>(h_all);
+    for (int s = 0; s < num_sources; ++s) {
+        const flr_source &q = sources[s];
+        if (q.num_pts < 1 || q.offset < 0 || q.projections_start_ind < 0 || q.tdi_start_ind < 0)
+            return fail(FLR_ERR_INVALID_ARGUMENT, "source " + std::to_string(s) + ": bad size or offset");
+        ProjSource &p = ps[s];
+        p.y = y_all + 6 * q.offset;
+        p.h = h2 + q.offset;
+        p.t_data = nullptr;
+        p.num_delays = p.num_inputs = q.num_pts;
+        p.start = q.projections_start_ind;
+        p.t0 = q.t0;
+        for (int c = 0; c < 3; ++c) {
+            p.k[c] = q.k[c];
+            p.u[c] = q.u[c];
+            p.v[c] = q.v[c];
+        }
+        TdiSource &t = ts[s];
+        t.out = out_all + 3 * q.offset;
+        t.y = p.y;
+        t.t_arr = nullptr;
+        t.num_inputs = t.num_delays = q.num_pts;
+        t.start = q.tdi_start_ind;
+        t.t0 = q.t0;
+        const int64_t np = (int64_t)pt[s] + tiles_for(q.num_pts, p.start, zero_edges);
+        const int64_t nt = (int64_t)tt[s] + tiles_for(q.num_pts, t.start, zero_edges);
+        if (np > INT32_MAX || nt > INT32_MAX) return fail(FLR_ERR_UNSUPPORTED, "batch has too many tiles");
+        pt[s + 1] = (int)np;
+        tt[s + 1] = (int)nt;
+        samples += q.num_pts;
+    }
+    auto b = new flr_batch();
+    b->r = r;
+    b->flags = flags;
+    b->dt = dt;
+    b->fs = sampling_frequency;
+    b->num_sources = num_sources;
+    b->samples = samples;
+    b->proj_tiles = pt[num_sources];
+    b->tdi_tiles = tt[num_sources];
+    hipError_t e = hipMalloc(&b->d_proj, sizeof(ProjSource) * num_sources);
+    if (e == hipSuccess) e = hipMalloc(&b->d_tdi, sizeof(TdiSource) * num_sources);
+    if (e == hipSuccess) e = hipMalloc(&b->d_proj_tiles, sizeof(int) * (num_sources + 1));
+    if (e == hipSuccess) e = hipMalloc(&b->d_tdi_tiles, sizeof(int) * (num_sources + 1));
+    if (e == hipSuccess) e = hipMemcpy(b->d_proj, ps.data(), sizeof(ProjSource) * num_sources, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->d_tdi, ts.data(), sizeof(TdiSource) * num_sources, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->d_proj_tiles, pt.data(), sizeof(int) * (num_sources + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->d_tdi_tiles, tt.data(), sizeof(int) * (num_sources + 1), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        flr_batch_destroy(b);
+        return hip_fail(e, "batch upload");
+    }
+    *out = b;
+    return FLR_OK;
+}
+
+int flr_batch_run(flr_batch *b, void *stream)
+{
+    if (!b || !b->r) return fail(FLR_ERR_INVALID_ARGUMENT, "batch is NULL");
+    flr_response *r = b->r;
+    DeviceGuard g(r->device);
+    hipStream_t s = (hipStream_t)stream;
+    const bool exact = (b->flags & FLR_FLAG_EXACT) != 0;
+    ProjConfig pc;
+    int st = make_proj_config(r, b->dt, b->fs, b->flags, pc);
+    if (st) return st;
+    TdiConfig tc;
+    StockTdi stock;
+    st = make_tdi_config(r, b->dt, b->fs, b->flags, tc, stock);
+    if (st) return st;
+    Batch<ProjSource> bp{b->d_proj, b->d_proj_tiles, b->num_sources, b->proj_tiles};
+    Batch<TdiSource> bt{b->d_tdi, b->d_tdi_tiles, b->num_sources, b->tdi_tiles};
+    hipError_t e = launch_projection_batch(pc, bp, r->orbits->view(), r->lagrange(), exact, s);
+    if (e != hipSuccess) return hip_fail(e, "batch projection launch");
+    e = launch_tdi_batch(tc, bt, r->orbits->view(), r->lagrange(), exact, stock, s);
+    if (e != hipSuccess) return hip_fail(e, "batch tdi launch");
+    return FLR_OK;
+}
+
+int64_t flr_batch_samples(const flr_batch *b) { return b ? b->samples : 0; }
 
 int flr_response_set_exact(flr_response *r, int exact)
 {

@@ -11,7 +11,7 @@ constexpr int kNLinks = 6;
 constexpr int kMaxHalf = 20;      // half point count h = (order+1)/2, order <= 39 (reference MAX_ORDER 40)
 constexpr int kMaxUnits = 256;    // TDI units (signed, delayed link terms)
 constexpr int kMaxChain = 4096;   // total delay-chain entries over all units
-constexpr int kBlock = 256;       // threads per workgroup (4 wave64)
+constexpr int kBlock = 256;       // threads per workgroup (4 wave64) = samples per tile
 
 // Orbit table, packed per node: 15 values [ltt row 0..5 | x sc1 xyz | sc2 xyz | sc3 xyz]
 // followed by their 15 slopes (v[k+1] - v[k]) / dt, so that one node's data sits in 240 B.
@@ -37,18 +37,12 @@ struct LagrangeView {
     double E[kMaxHalf];     // E[j-1], j = 1..h-1
 };
 
-// Projection: each spacecraft s (0..2) receives two links; rec_row[s][q] is the table row
-// (and y_gw row) of the q-th link received by s and em_sc[s][q] its emitter spacecraft.
-struct ProjParams {
-    double *y;              // [6][num_delays]
-    const double2 *h;       // [num_inputs] h+ + i hx
-    const double *t_data;   // [num_delays] or nullptr (uniform grid)
-    int num_delays;
-    int num_inputs;
-    int start;              // projections_start_ind
-    int grid_begin;         // first sample covered by tile 0
-    double t0, dt, fs;      // uniform grid t_i = t0 + i*dt
-    double k[3], u[3], v[3];
+// ------------------------------------------------------------------ projection
+// Shared by every source of a launch. Each spacecraft s (0..2) receives two links;
+// rec_row[s][q] is the table row (and y row) of the q-th link received by s, em_sc[s][q] its
+// emitter spacecraft.
+struct ProjConfig {
+    double dt, fs;
     int rec_row[3][2];
     int em_sc[3][2];
     int halo_lo, halo_hi;   // staged window = [first - halo_lo, last + halo_hi]
@@ -57,23 +51,30 @@ struct ProjParams {
     int *err;
 };
 
+// One source: t_i = t0 + i*dt unless t_data is given.
+struct ProjSource {
+    double *y;              // [6][num_delays]
+    const double2 *h;       // [num_inputs] h+ + i hx
+    const double *t_data;   // [num_delays] or nullptr
+    int num_delays;
+    int num_inputs;
+    int start;              // projections_start_ind
+    int pad;
+    double t0;
+    double k[3], u[3], v[3];
+};
+
+// ------------------------------------------------------------------ TDI
 struct TdiUnit {
     double sign;
-    int base_row;           // y_gw row of the delayed link
+    int base_row;           // y row of the delayed link
     int chain_start;        // into chain_rows
     int chain_len;
     int pad;
 };
 
-struct TdiParams {
-    double *out;            // [3][num_delays]
-    const double *y;        // [6][num_inputs]
-    const double *t_arr;    // [num_delays] or nullptr
-    int num_inputs;
-    int num_delays;
-    int start;              // tdi_start_ind
-    int grid_begin;
-    double t0, dt, fs;
+struct TdiConfig {
+    double dt, fs;
     const TdiUnit *units;   // sorted by channel, index order kept within a channel
     const int *chain_rows;  // orbit-table rows, -1 for the undelayed sentinel
     int ch_begin[4];        // units [ch_begin[c], ch_begin[c+1]) feed channel c
@@ -86,7 +87,18 @@ struct TdiParams {
     int *err;
 };
 
-// Device row (orbit-table column / y_gw row) of each link in the canonical order
+struct TdiSource {
+    double *out;            // [3][num_delays]
+    const double *y;        // [6][num_inputs]
+    const double *t_arr;    // [num_delays] or nullptr
+    int num_inputs;
+    int num_delays;
+    int start;              // tdi_start_ind
+    int pad;
+    double t0;
+};
+
+// Device row (orbit-table column / y row) of each link in the canonical order
 // 12, 23, 31, 13, 32, 21.
 struct RowMap {
     int row[6];
@@ -100,11 +112,36 @@ struct StockTdi {
     RowMap rows;
 };
 
+// A batch: sources[num_sources] and tile_start[num_sources + 1] (prefix sums of the tiles of
+// each source) in device memory.
+template <typename Source>
+struct Batch {
+    const Source *sources;
+    const int *tile_start;
+    int num_sources;
+    int total_tiles;
+};
+
+// Tiles a source needs: one per kBlock samples of [start, n - start), or of [0, n) when the
+// edges are zeroed too.
+inline int tiles_for(int num_delays, int start, int zero_edges)
+{
+    const int span = zero_edges ? num_delays : num_delays - 2 * start;
+    return span > 0 ? (span + kBlock - 1) / kBlock : 0;
+}
+
 // Launchers (flr_kernels.hip).
-hipError_t launch_projection(const ProjParams &p, const OrbitView &orb, const LagrangeView &lg,
-                             bool exact, hipStream_t stream);
-hipError_t launch_tdi(const TdiParams &p, const OrbitView &orb, const LagrangeView &lg, bool exact,
-                      const StockTdi &stock, hipStream_t stream);
+hipError_t launch_projection(const ProjConfig &cfg, const ProjSource &src, const OrbitView &orb,
+                             const LagrangeView &lg, bool exact, hipStream_t stream);
+hipError_t launch_projection_batch(const ProjConfig &cfg, const Batch<ProjSource> &batch,
+                                   const OrbitView &orb, const LagrangeView &lg, bool exact,
+                                   hipStream_t stream);
+hipError_t launch_tdi(const TdiConfig &cfg, const TdiSource &src, const OrbitView &orb,
+                      const LagrangeView &lg, bool exact, const StockTdi &stock,
+                      hipStream_t stream);
+hipError_t launch_tdi_batch(const TdiConfig &cfg, const Batch<TdiSource> &batch,
+                            const OrbitView &orb, const LagrangeView &lg, bool exact,
+                            const StockTdi &stock, hipStream_t stream);
 hipError_t launch_orbit_eval(const OrbitView &orb, const double *t, int n, int row, int sc,
                              double *ltt_out, double *pos_out, hipStream_t stream);
 hipError_t launch_clear(int *word, hipStream_t stream);

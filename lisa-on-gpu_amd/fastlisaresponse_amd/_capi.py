@@ -43,8 +43,27 @@ SIGNATURES = {
     "flr_projection": (_ci, [_vp, _vp, _vp, _ci, _vp, _cd, _cd, _cd, _vp, _vp, _vp, _ci, _ci, _vp]),
     "flr_tdi": (_ci, [_vp, _vp, _vp, _ci, _vp, _cd, _cd, _cd, _ci, _ci, _vp]),
     "flr_check_errors": (_ci, [_vp, _vp]),
+    "flr_batch_create": (_ci, [_vp, _vp, _ci, _vp, _vp, _vp, _cd, _cd, _ci, ctypes.POINTER(_vp)]),
+    "flr_batch_run": (_ci, [_vp, _vp]),
+    "flr_batch_samples": (ctypes.c_int64, [_vp]),
+    "flr_batch_destroy": (None, [_vp]),
     "flr_response_set_exact": (_ci, [_vp, _ci]),
 }
+
+
+class FlrSource(ctypes.Structure):
+    """flr_source of include/flr.h: one source of a packed batch."""
+    _fields_ = [
+        ("offset", ctypes.c_int64),
+        ("num_pts", ctypes.c_int32),
+        ("projections_start_ind", ctypes.c_int32),
+        ("tdi_start_ind", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("t0", ctypes.c_double),
+        ("k", ctypes.c_double * 3),
+        ("u", ctypes.c_double * 3),
+        ("v", ctypes.c_double * 3),
+    ]
 
 _lib = None
 
