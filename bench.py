@@ -40,7 +40,6 @@ HBM_PEAK_GBS = 8000.0
 TAPS = ORDER + 1
 FLOP_TDI = {"1st generation": 2 * TAPS * 24, "2nd generation": 2 * TAPS * 48}
 FLOP_PROJ = 2 * TAPS * 2 * 2 * 6  # 6 links x (rec, em) x (h+, hx)
-PROFILE_PMC = os.path.join(ROOT, "profiles", "tdi_kernel_1yr_2nd_gen_fp64_pmc.txt")
 
 
 def gb_waveform(n, dt, A=1e-21, f=5e-3, fdot=1e-16, iota=0.8, phi0=0.3, psi=0.5):
@@ -56,10 +55,14 @@ def gb_waveform(n, dt, A=1e-21, f=5e-3, fdot=1e-16, iota=0.8, phi0=0.3, psi=0.5)
     return hp + 1j * hc
 
 
-def pmc_traffic():
-    """HBM bytes per TDI launch from the committed rocprofv3 --pmc summary, or None."""
+def pmc_traffic(form: str, tdi: str, orbits: str):
+    """HBM bytes per TDI launch from the committed rocprofv3 --pmc summary of this workload
+    (FETCH_SIZE and WRITE_SIZE, calibrated against a copy kernel), or None if none exists."""
+    gen = "2nd_gen" if tdi.startswith("2nd") else "1st_gen"
+    arms = "" if orbits == "equal" else "_unequal"
+    path = os.path.join(ROOT, "profiles", f"kernels_1yr_{gen}_fp64_{form}{arms}_pmc.txt")
     try:
-        with open(PROFILE_PMC) as f:
+        with open(path) as f:
             m = re.search(r"^HBM_BYTES_PER_LAUNCH\s+([0-9.eE+]+)", f.read(), re.M)
         return float(m.group(1)) if m else None
     except OSError:
@@ -93,6 +96,9 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=1 << 21,
                     help="samples in the CPU-baseline sample (0 skips it)")
     ap.add_argument("--no-parity", action="store_true", help="skip the full-size oracle check")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="also run one elementwise FP64 kernel reading and writing BYTES each "
+                         "(calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE against a known size)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -151,7 +157,8 @@ def main() -> int:
     achieved = flop * N_SAMPLES / (kms * 1e-3) / 1e12
     roofline = {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                "traffic": pmc_traffic() if kname == "tdi_kernel" else None,
+                "traffic": (pmc_traffic("fast" if args.fast else "exact", args.tdi, args.orbits)
+                            if kname == "tdi_kernel" else None),
                 "kernel": kname, "kernel_ms": kms,
                 "algorithmic_flop_per_sample": flop}
 
@@ -180,6 +187,13 @@ def main() -> int:
         mabs = [float(np.abs(got[c] - aet_ref[c]).max()) for c in range(3)]
         line["parity"] = {"vs": "FP64 CPU oracle, full workload", "rel_l2_AET": rel,
                           "max_abs_AET": mabs, "bitwise_equal": bool(np.array_equal(got, aet_ref))}
+    if args.copy_probe > 0:
+        src = torch.ones(args.copy_probe // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        line["copy_probe_bytes"] = args.copy_probe
     if args.cpu_sample > 0:
         line["cpu_baseline"] = cpu_baseline(orbits, h_host, lam, beta, args.tdi, args.cpu_sample)
     print(json.dumps(line), flush=True)
