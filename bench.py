@@ -1,18 +1,25 @@
 #!/usr/bin/env python3
-"""Throughput of the LISA time-domain response on one MI355X.
+"""Throughput of the LISA time-domain response on one MI355X: mixed precision vs FP64.
 
-Workload: one galactic-binary source over one year, N = 3,153,600 samples at dt = 10 s,
-order-25 Lagrange stencils, 2nd-generation TDI, A/E/T output, FP64 throughout, equal-arm
-analytic cartwheel orbits tabulated every 1e4 s. A step is one projection + TDI(+AET) pass over
-the source with its inputs already resident in HBM; `value` is AET samples/s.
+Workload: one galactic-binary source over one year, N = 3,153,600 samples at dt = 10 s
+(f0 = 5 mHz, fdot = 1e-16), order-25 Lagrange stencils, 2nd-generation TDI, A/E/T output,
+analytic equal-arm cartwheel orbits tabulated every 1e4 s. A step is one pass of the response
+over the source with its input already resident in HBM; `value` is AET samples/s of the
+mixed-precision form: complex64 input h / max|h|, one fused projection + TDI launch (the six
+projections stay in LDS), float32 A, E, T out; times, delays and stencil arithmetic in FP64.
 
-Also reported on the same JSON line:
-  roofline      dominant kernel's algorithmic FP64 rate (stencil multiply-adds) against the
-                78.6 TFLOP/s FP64 peak, its launch time from HIP events in the timed region,
-                and HBM bytes per launch from the committed rocprofv3 counter summary
+Also on the same JSON line:
+  fp64          the plain FP64 drop-in (separate projection and TDI launches, exact stencils,
+                bitwise equal to the CPU oracle) timed in the same run, and mixed / fp64
+  roofline      the fused mixed-precision kernel's algorithmic FP64 rate (stencil
+                multiply-adds, 3744 FLOP per 2nd-generation sample) against the 78.6 TFLOP/s
+                FP64 vector peak, its launch time from HIP events in the timed region, and HBM
+                bytes per launch from the committed rocprofv3 counter summary
+  latency       p50/p99 of a short open-loop serving run: Poisson arrivals at 200 req/s of
+                1..64-source requests (2^18/2^20/2^22 samples, dt 5/10 s, 30/70 1st/2nd gen)
+  parity        rel-L2 / max-abs of A, E, T against the FP64 oracle on the full workload
   cpu_baseline  the FP64 CPU oracle (1 thread, the reference CPU path's serial arithmetic) on
                 a bounded prefix of the same workload
-  parity        rel-L2 / max-abs of A, E, T against the FP64 oracle on the full workload
 """
 from __future__ import annotations
 
@@ -36,10 +43,15 @@ ORDER = 25
 T_BUFFER = 10000.0
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
-# Algorithmic FLOPs per output sample: 2 * (stencil taps) per stencil evaluation.
 TAPS = ORDER + 1
+# Algorithmic FLOPs per output sample: 2 per stencil tap. Projection: 6 links x (receiver,
+# emitter) x (h+, hx); TDI: one real stencil per unit (24 or 48 units).
+FLOP_PROJ = 2 * TAPS * 6 * 2 * 2
 FLOP_TDI = {"1st generation": 2 * TAPS * 24, "2nd generation": 2 * TAPS * 48}
-FLOP_PROJ = 2 * TAPS * 2 * 2 * 6  # 6 links x (rec, em) x (h+, hx)
+# Algorithmic HBM bytes per sample: h in, A, E, T out (y stays on chip in the fused kernel);
+# the FP64 drop-in also writes and reads the six projections.
+BYTES_MIXED = 8 + 12
+BYTES_FP64 = 16 + 48 + 48 + 24
 
 
 def gb_waveform(n, dt, A=1e-21, f=5e-3, fdot=1e-16, iota=0.8, phi0=0.3, psi=0.5):
@@ -55,12 +67,10 @@ def gb_waveform(n, dt, A=1e-21, f=5e-3, fdot=1e-16, iota=0.8, phi0=0.3, psi=0.5)
     return hp + 1j * hc
 
 
-def pmc_traffic(form: str, tdi: str, orbits: str):
-    """HBM bytes per TDI launch from the committed rocprofv3 --pmc summary of this workload
-    (FETCH_SIZE and WRITE_SIZE, calibrated against a copy kernel), or None if none exists."""
-    gen = "2nd_gen" if tdi.startswith("2nd") else "1st_gen"
-    arms = "" if orbits == "equal" else "_unequal"
-    path = os.path.join(ROOT, "profiles", f"kernels_1yr_{gen}_fp64_{form}{arms}_pmc.txt")
+def pmc_traffic(name: str):
+    """HBM bytes per launch from a committed rocprofv3 --pmc summary (FETCH_SIZE and
+    WRITE_SIZE, calibrated against a copy kernel), or None if it does not exist."""
+    path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as f:
             m = re.search(r"^HBM_BYTES_PER_LAUNCH\s+([0-9.eE+]+)", f.read(), re.M)
@@ -71,9 +81,9 @@ def pmc_traffic(form: str, tdi: str, orbits: str):
 
 def cpu_baseline(orbits, h, lam, beta, tdi, n_sample):
     """Oracle projection + TDI + AET on the first n_sample samples, one thread."""
-    import numpy as np
     from oracle import oracle as orc
     oo = orc.OracleOrbits(*orbits.pycppdetector_args)
+    import numpy as np
     hs = np.ascontiguousarray(h[:n_sample])
     t0 = time.perf_counter()
     orc.response_pipeline(oo, hs, lam, beta, DT, ORDER, tdi, x_base=orbits.x_base, nthreads=1)
@@ -83,19 +93,68 @@ def cpu_baseline(orbits, h, lam, beta, tdi, n_sample):
                       f"({el:.1f} s, FP64 oracle restating the reference CPU path)"}
 
 
+def timed(fn, steps, warmup):
+    """Warm up, then time `steps` calls with HIP events on the current stream and the wall
+    clock (device synchronised on both sides). Returns (wall s, mean event ms per call)."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    t0 = time.perf_counter()
+    for a, b in ev:
+        a.record()
+        fn()
+        b.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, sum(a.elapsed_time(b) for a, b in ev) / steps
+
+
+def serving_run(seconds, rate, seed=5):
+    import numpy as np
+    from fastlisaresponse_amd.orbits import EqualArmlengthOrbits
+    from fastlisaresponse_amd.serving import ResponseServer, gb_pool, poisson_requests, run_open_loop
+    orb = EqualArmlengthOrbits(t_max=1.5 * YRSID_SI)
+    orb.configure(linear_interp_setup=True)
+    keys = [(dt, g) for dt in (5.0, 10.0) for g in ("1st generation", "2nd generation")]
+    server = ResponseServer(orb, keys, hbm_budget=64 << 30, num_streams=4)
+    pool = gb_pool(8, 1 << 22, 10.0, seed=seed)
+    # warm-up: one small request per key, outside the measurement
+    wt, wr = poisson_requests(np.random.default_rng(seed + 1), 50.0, 0.2, pool,
+                              lengths=(1 << 18,), max_sources=4)
+    run_open_loop(server, wt, wr)
+    server.done.clear()
+    server.busy_intervals.clear()
+    server.launches = 0
+    times, reqs = poisson_requests(np.random.default_rng(seed), rate, seconds, pool)
+    stats = run_open_loop(server, times, reqs, drain_timeout=300.0)
+    server.close()
+    stats.pop("queue_trace", None)
+    stats.update({"arrival_rate_per_s": rate, "arrival_seconds": seconds,
+                  "workload": "Poisson arrivals, 1-64 sources/request, N in {2^18, 2^20, 2^22}, "
+                              "dt in {5, 10} s, 30/70 1st/2nd gen, mixed precision, 4 streams, "
+                              "64 GiB admission budget"})
+    return stats
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--tdi", default="2nd generation", choices=list(FLOP_TDI))
-    ap.add_argument("--fast", action="store_true",
-                    help="reciprocal-free stencils instead of the (default) bitwise-exact form")
     ap.add_argument("--orbits", default="equal", choices=["equal", "unequal"],
                     help="equal-arm cartwheel (default) or the unequal-arm cartwheel")
     ap.add_argument("--cpu-sample", type=int, default=1 << 21,
                     help="samples in the CPU-baseline sample (0 skips it)")
     ap.add_argument("--no-parity", action="store_true", help="skip the full-size oracle check")
+    ap.add_argument("--serve-seconds", type=float, default=2.0,
+                    help="seconds of Poisson arrivals in the serving run (0 skips it)")
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="requests per second")
+    ap.add_argument("--only", choices=["mixed", "fp64"], default=None,
+                    help="time one form only (for profiling)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="also run one elementwise FP64 kernel reading and writing BYTES each "
                          "(calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE against a known size)")
@@ -110,6 +169,7 @@ def main() -> int:
     import numpy as np
     import torch
     from fastlisaresponse_amd import pyResponseTDI
+    from fastlisaresponse_amd.batch import BatchedResponse, pack_mixed
     from fastlisaresponse_amd.orbits import CartwheelOrbits, EqualArmlengthOrbits
 
     torch.cuda.set_device(0)
@@ -117,64 +177,61 @@ def main() -> int:
     orbits = orbit_cls(t_max=2.0 * YRSID_SI)
     orbits.configure(linear_interp_setup=True)
     lam, beta = 1.2, 0.3
-    r = pyResponseTDI(1 / DT, N_SAMPLES, order=ORDER, tdi=args.tdi, orbits=orbits, tdi_chan="AET",
-                      exact=not args.fast)
     h_host = gb_waveform(N_SAMPLES, DT)
     h = torch.as_tensor(h_host, device="cuda")
+    flop = FLOP_PROJ + FLOP_TDI[args.tdi]
+    line = {"metric": METRIC, "unit": "samples/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "storage": "f32 (h, A/E/T)",
+            "data": "synthetic (one galactic binary, f0=5 mHz, fdot=1e-16; analytic "
+                    + ("equal-arm" if args.orbits == "equal" else "unequal-arm") + " orbits)",
+            "config": {"workload": f"1-yr order-25 {args.tdi} TDI AET, mixed precision fused "
+                                   "launch vs plain FP64 drop-in",
+                       "num_pts": N_SAMPLES, "dt": DT, "order": ORDER, "tdi": args.tdi,
+                       "tdi_chan": "AET", "orbits": args.orbits + "-arm cartwheel"}}
 
-    def step(ev=None):
-        if ev is not None:
-            ev[0].record()
-        r.get_projections(h, lam, beta, t0=0.0, t_buffer=T_BUFFER)
-        if ev is not None:
-            ev[1].record()
-        out = r.get_tdi_delays()
-        if ev is not None:
-            ev[2].record()
-        return out
+    out_mixed = out_fp64 = None
+    if args.only in (None, "mixed"):
+        br = BatchedResponse(1 / DT, tdi=args.tdi, orbits=orbits, tdi_chan="AET",
+                             precision="mixed")
+        h32, scales = pack_mixed(h, [N_SAMPLES])
+        plan = br.plan(h32, [N_SAMPLES], [lam], [beta], scales=scales)
+        wall, ms = timed(plan.run, args.steps, args.warmup)
+        plan.check_errors()
+        out_mixed = plan.out_all.view(3, -1)
+        line.update({"value": N_SAMPLES * args.steps / wall, "ms_per_step": wall * 1e3 / args.steps})
+        achieved = flop * N_SAMPLES / (ms * 1e-3) / 1e12
+        traffic = pmc_traffic(f"fused_mixed_1yr_{'2nd' if args.tdi.startswith('2nd') else '1st'}"
+                              f"_gen_{args.orbits}_pmc.txt")
+        line["roofline"] = {
+            "bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+            "kernel": "fused_batch_kernel (mixed)", "kernel_ms": ms,
+            "algorithmic_flop_per_sample": flop, "algorithmic_bytes_per_sample": BYTES_MIXED,
+            "hbm_achieved_GBs": BYTES_MIXED * N_SAMPLES / (ms * 1e-3) / 1e9,
+            "hbm_frac": BYTES_MIXED * N_SAMPLES / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
-    for _ in range(args.warmup):
-        step()
-    r.check_errors()
-    torch.cuda.synchronize()
+    if args.only in (None, "fp64"):
+        r = pyResponseTDI(1 / DT, N_SAMPLES, order=ORDER, tdi=args.tdi, orbits=orbits,
+                          tdi_chan="AET")
+        res = {}
 
-    events = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    for k in range(args.steps):
-        out = step(events[k])
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t_start
-    r.check_errors()
+        def fp64_step():
+            r.get_projections(h, lam, beta, t0=0.0, t_buffer=T_BUFFER)
+            res["out"] = r.get_tdi_delays()
 
-    proj_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in events]))
-    tdi_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in events]))
-    value = N_SAMPLES * args.steps / elapsed
-    if tdi_ms >= proj_ms:
-        kname, kms, flop = "tdi_kernel", tdi_ms, FLOP_TDI[args.tdi]
-    else:
-        kname, kms, flop = "projection_kernel", proj_ms, FLOP_PROJ
-    achieved = flop * N_SAMPLES / (kms * 1e-3) / 1e12
-    roofline = {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                "traffic": (pmc_traffic("fast" if args.fast else "exact", args.tdi, args.orbits)
-                            if kname == "tdi_kernel" else None),
-                "kernel": kname, "kernel_ms": kms,
-                "algorithmic_flop_per_sample": flop}
-
-    line = {
-        "metric": METRIC, "value": value, "unit": "samples/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic (one galactic binary, f0=5 mHz, fdot=1e-16; analytic equal-arm orbits)",
-        "config": {"workload": f"1-yr order-25 {args.tdi} TDI AET, FP64 plain GPU form",
-                   "num_pts": N_SAMPLES, "dt": DT, "order": ORDER, "tdi": args.tdi,
-                   "tdi_chan": "AET", "stencil_form": "fast" if args.fast else "exact",
-                   "orbits": "equal-arm cartwheel" if args.orbits == "equal"
-                   else "unequal-arm cartwheel"},
-        "kernels_ms": {"projection_kernel": proj_ms, "tdi_kernel": tdi_ms},
-        "roofline": roofline,
-    }
+        wall64, ms64 = timed(fp64_step, args.steps, args.warmup)
+        r.check_errors()
+        out_fp64 = torch.stack(res["out"])
+        fp64 = {"value": N_SAMPLES * args.steps / wall64, "ms_per_step": wall64 * 1e3 / args.steps,
+                "form": "plain FP64 drop-in: projection + TDI launches, exact stencils",
+                "algorithmic_bytes_per_sample": BYTES_FP64,
+                "achieved_TFLOPs": flop * N_SAMPLES / (ms64 * 1e-3) / 1e12}
+        line["fp64"] = fp64
+        if args.only == "fp64":
+            line.update({"value": fp64["value"], "ms_per_step": fp64["ms_per_step"]})
+        elif "value" in line:
+            line["mixed_vs_fp64"] = line["value"] / fp64["value"]
 
     if not args.no_parity:
         from oracle import oracle as orc
@@ -182,11 +239,24 @@ def main() -> int:
         nth = max(1, min(16, os.cpu_count() or 1))
         _, aet_ref = orc.response_pipeline(oo, h_host, lam, beta, DT, ORDER, args.tdi,
                                            x_base=orbits.x_base, nthreads=nth)
-        got = torch.stack(out).cpu().numpy()
-        rel = [float(np.linalg.norm(got[c] - aet_ref[c]) / np.linalg.norm(aet_ref[c])) for c in range(3)]
-        mabs = [float(np.abs(got[c] - aet_ref[c]).max()) for c in range(3)]
-        line["parity"] = {"vs": "FP64 CPU oracle, full workload", "rel_l2_AET": rel,
-                          "max_abs_AET": mabs, "bitwise_equal": bool(np.array_equal(got, aet_ref))}
+        par = {"vs": "FP64 CPU oracle, full workload"}
+        T = int(T_BUFFER / DT)
+        for name, out in (("mixed", out_mixed), ("fp64", out_fp64)):
+            if out is None:
+                continue
+            got = out.double().cpu().numpy()
+            par[name] = {
+                "rel_l2_AET": [float(np.linalg.norm(got[c, T:-T] - aet_ref[c, T:-T]) /
+                                     np.linalg.norm(aet_ref[c, T:-T])) for c in range(3)],
+                "max_abs_AET": [float(np.abs(got[c] - aet_ref[c]).max()) for c in range(3)],
+                "bitwise_equal": bool(np.array_equal(got, aet_ref))}
+        line["parity"] = par
+
+    if args.serve_seconds > 0 and args.only is None:
+        s = serving_run(args.serve_seconds, args.serve_rate)
+        line["latency"] = {"p50_ms": s["p50_ms"], "p99_ms": s["p99_ms"]}
+        line["serving"] = s
+
     if args.copy_probe > 0:
         src = torch.ones(args.copy_probe // 8, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)

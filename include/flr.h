@@ -38,6 +38,7 @@ extern "C" {
 #define FLR_FLAG_ZERO_EDGES 0x4 /* also write 0 to samples outside the computed range */
 #define FLR_FLAG_OVERWRITE 0x8  /* TDI: store the sums instead of adding them to the output */
 #define FLR_FLAG_AET 0x10       /* TDI: store A, E, T (implies OVERWRITE) instead of X, Y, Z */
+#define FLR_FLAG_MIXED 0x20     /* fused batches: complex64 input (h / scale), float32 output */
 
 typedef struct flr_orbits flr_orbits;
 typedef struct flr_response flr_response;
@@ -166,6 +167,8 @@ typedef struct {
     int32_t reserved;
     double t0;                     /* time of the source's first sample (s) */
     double k[3], u[3], v[3];       /* sky propagation and polarisation basis */
+    double scale;                  /* fused batches: outputs are scale * response of the stored
+                                      input (0 means 1); ignored by flr_batch_create */
 } flr_source;
 
 typedef struct flr_batch flr_batch;
@@ -176,8 +179,39 @@ typedef struct flr_batch flr_batch;
 int flr_batch_create(flr_response *response, const flr_source *sources, int num_sources,
                      const double *h_all, double *y_all, double *out_all, double dt,
                      double sampling_frequency, int flags, flr_batch **out);
-/* Enqueue the projection and the TDI launches of the batch on `stream`. */
+/* Fused batch: no counterpart in the reference. One launch computes projection and TDI tile
+ * by tile with the six projections kept on chip (y is never written to memory), for order 25.
+ * FLR_FLAG_EXACT selects the exact stencils (FP64 only; then the outputs are bitwise those of
+ * flr_batch_create + flr_batch_run), otherwise the fast form. With FLR_FLAG_MIXED, h_all holds
+ * complex64 samples h / scale (scale per source) and out_all float32 outputs; without it h_all
+ * is complex128 and out_all double. Outputs are overwritten (X, Y, Z, or A, E, T with
+ * FLR_FLAG_AET); FLR_FLAG_ZERO_EDGES also zeroes samples outside [tdi_start_ind,
+ * num_pts - tdi_start_ind). Run and destroy it with flr_batch_run / flr_batch_destroy. */
+int flr_fused_create(flr_response *response, const flr_source *sources, int num_sources,
+                     const void *h_all, void *out_all, double dt, double sampling_frequency,
+                     int flags, flr_batch **out);
+/* Fused batch with room for `capacity` sources and none bound yet (device and pinned host
+ * tables are allocated here, once). Same flags as flr_fused_create. For serving loops, which
+ * must not allocate or synchronise per request. */
+int flr_fused_reserve(flr_response *response, int capacity, double dt, double sampling_frequency,
+                      int flags, flr_batch **out);
+/* Bind num_sources <= capacity sources (packed as for flr_fused_create) to a reserved fused
+ * batch: validates them on the host, then uploads the source table on `stream` without
+ * blocking (from pinned staging; a bind first waits for the previous bind's upload). A launch
+ * still running from an earlier bind must be ordered before this upload: bind and run a batch
+ * on one stream. */
+int flr_fused_bind(flr_batch *batch, const flr_source *sources, int num_sources,
+                   const void *h_all, void *out_all, void *stream);
+/* Enqueue the batch's launches on `stream` (projection then TDI, or the one fused launch). */
 int flr_batch_run(flr_batch *batch, void *stream);
+/* Launch shape of a batch (no counterpart in the reference): tile_samples window samples per
+ * workgroup tile (fused mixed: 256, 512 or 1024; fused FP64: 512; unfused: 256; 0 keeps the
+ * current; a fused tile outputs tile_samples minus the TDI delay halos, so that its projection
+ * window is whole rounds of the workgroup) and grid_per_cu persistent workgroups per CU
+ * (1..64; 0 keeps the current, default 8). Results do not depend on either. Call it when no
+ * launch of the batch is in flight; the autotuner (fastlisaresponse_amd.autotune) times the
+ * candidates and keeps the fastest. */
+int flr_batch_set_launch(flr_batch *batch, int tile_samples, int grid_per_cu);
 /* Total samples of the batch. */
 int64_t flr_batch_samples(const flr_batch *batch);
 void flr_batch_destroy(flr_batch *batch);

@@ -89,7 +89,7 @@ struct flr_orbits {
             if (links[r] == link) return r;
         return -1;
     }
-    OrbitView view() const { return OrbitView{d_table, dt, N}; }
+    OrbitView view() const { return OrbitView{d_table, dt, N, 1.0 / dt}; }
 };
 
 struct flr_response {
@@ -144,6 +144,17 @@ struct flr_batch {
     int *d_proj_tiles = nullptr;  // [num_sources + 1] prefix sums
     int *d_tdi_tiles = nullptr;
     int proj_tiles = 0, tdi_tiles = 0;
+    bool fused = false;           // one fused launch over d_fused / d_fused_tiles
+    FusedSource *d_fused = nullptr;
+    int *d_fused_tiles = nullptr;
+    int fused_tiles = 0;
+    int capacity = 0;             // fused: sources the tables hold
+    int tile_passes = kFusedPasses;  // fused: the tile's window is tile_passes * kBlock samples
+    int tile_len = 0;                // fused: output samples per tile (make_fused_config)
+    int grid_per_cu = 8;             // persistent workgroups per CU
+    FusedSource *h_fused = nullptr;  // pinned staging of the source table and tile prefix
+    int *h_fused_tiles = nullptr;
+    hipEvent_t uploaded = nullptr;   // the last bind's upload
 };
 
 namespace {
@@ -373,6 +384,28 @@ int make_tdi_config(const flr_response *r, double dt, double fs, int flags, TdiC
     stock.equal_arm = o->equal_arm;
     const int canon_links[6] = {12, 23, 31, 13, 32, 21};
     for (int k = 0; k < 6; ++k) stock.rows.row[k] = o->row_of(canon_links[k]);
+    return FLR_OK;
+}
+
+// Fused launch: the TDI window of a tile of passes * kBlock samples, and the h window that
+// projects it.
+int make_fused_config(const flr_response *r, double dt, double fs, int flags, int passes,
+                      ProjConfig &pc, TdiConfig &tc, StockTdi &stock)
+{
+    int st = make_proj_config(r, dt, fs, flags, pc);
+    if (st) return st;
+    if ((st = make_tdi_config(r, dt, fs, flags | FLR_FLAG_OVERWRITE, tc, stock))) return st;
+    // mixed precision: room for the composite filter's kFir taps after the last sample
+    if (flags & FLR_FLAG_MIXED) tc.halo_hi = std::max(tc.halo_hi, kFir - (r->order + 1) / 2 + 1);
+    // tile + halos = passes * kBlock: every projection round uses the whole workgroup
+    const int halos = tc.halo_lo + tc.halo_hi;
+    tc.tile = passes * kBlock - halos >= (passes * kBlock) / 2 ? passes * kBlock - halos
+                                                               : passes * kBlock;
+    tc.win_cap = tc.tile + halos;
+    pc.win_cap = tc.win_cap + pc.halo_lo + pc.halo_hi;
+    const bool generic = !(stock.generation != 0 && stock.equal_arm);
+    if (fused_lds_bytes(pc, tc, generic) > kMaxLds)
+        return fail(FLR_ERR_UNSUPPORTED, "fused delay windows exceed LDS (sampling too fine)");
     return FLR_OK;
 }
 
@@ -738,6 +771,14 @@ void flr_batch_destroy(flr_batch *b)
     if (b->d_tdi) (void)hipFree(b->d_tdi);
     if (b->d_proj_tiles) (void)hipFree(b->d_proj_tiles);
     if (b->d_tdi_tiles) (void)hipFree(b->d_tdi_tiles);
+    if (b->uploaded) {
+        (void)hipEventSynchronize(b->uploaded);
+        (void)hipEventDestroy(b->uploaded);
+    }
+    if (b->d_fused) (void)hipFree(b->d_fused);
+    if (b->d_fused_tiles) (void)hipFree(b->d_fused_tiles);
+    if (b->h_fused) (void)hipHostFree(b->h_fused);
+    if (b->h_fused_tiles) (void)hipHostFree(b->h_fused_tiles);
     delete b;
 }
 
@@ -814,6 +855,123 @@ int flr_batch_create(flr_response *r, const flr_source *sources, int num_sources
     return FLR_OK;
 }
 
+int flr_fused_reserve(flr_response *r, int capacity, double dt, double sampling_frequency,
+                      int flags, flr_batch **out)
+{
+    if (!out) return fail(FLR_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    if (!r || !r->orbits) return fail(FLR_ERR_NO_ORBITS, "Must add orbits with add_orbit_information method.");
+    if (!r->lag_set || !r->tdi_set) return fail(FLR_ERR_INVALID_ARGUMENT, "Lagrange and TDI tables must be set");
+    if (r->order != 25) return fail(FLR_ERR_UNSUPPORTED, "fused batches are built for order 25");
+    if ((flags & FLR_FLAG_MIXED) && (flags & FLR_FLAG_EXACT))
+        return fail(FLR_ERR_UNSUPPORTED, "FLR_FLAG_EXACT needs FP64 storage (no FLR_FLAG_MIXED)");
+    if (capacity < 1) return fail(FLR_ERR_INVALID_ARGUMENT, "capacity must be >= 1");
+    if (!(sampling_frequency > 0.0) || !(dt > 0.0)) return fail(FLR_ERR_INVALID_ARGUMENT, "dt and sampling_frequency must be > 0");
+    DeviceGuard g(r->device);
+    ProjConfig pc;
+    TdiConfig tc;
+    StockTdi stock;
+    int st = make_fused_config(r, dt, sampling_frequency, flags, kFusedPasses, pc, tc, stock);
+    if (st) return st;
+    auto b = new flr_batch();
+    b->r = r;
+    b->flags = flags;
+    b->dt = dt;
+    b->fs = sampling_frequency;
+    b->fused = true;
+    b->capacity = capacity;
+    b->tile_len = tc.tile;
+    hipError_t e = hipMalloc(&b->d_fused, sizeof(FusedSource) * capacity);
+    if (e == hipSuccess) e = hipMalloc(&b->d_fused_tiles, sizeof(int) * (capacity + 1));
+    if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_fused, sizeof(FusedSource) * capacity);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_fused_tiles, sizeof(int) * (capacity + 1));
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->uploaded, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        flr_batch_destroy(b);
+        return hip_fail(e, "fused batch reserve");
+    }
+    *out = b;
+    return FLR_OK;
+}
+
+int flr_fused_bind(flr_batch *b, const flr_source *sources, int num_sources, const void *h_all,
+                   void *out_all, void *stream)
+{
+    if (!b || !b->r || !b->fused) return fail(FLR_ERR_INVALID_ARGUMENT, "not a fused batch");
+    if (num_sources < 1 || !sources) return fail(FLR_ERR_INVALID_ARGUMENT, "no sources");
+    if (num_sources > b->capacity)
+        return fail(FLR_ERR_INVALID_ARGUMENT, "more sources than the batch was reserved for");
+    if (!h_all || !out_all) return fail(FLR_ERR_INVALID_ARGUMENT, "NULL packed array");
+    DeviceGuard g(b->r->device);
+    const int flags = b->flags;
+    const int zero_edges = (flags & FLR_FLAG_ZERO_EDGES) ? 1 : 0;
+    const size_t hsz = (flags & FLR_FLAG_MIXED) ? 2 * sizeof(float) : 2 * sizeof(double);
+    const size_t osz = (flags & FLR_FLAG_MIXED) ? sizeof(float) : sizeof(double);
+    std::vector<FusedSource> fs(num_sources);
+    std::vector<int> ft(num_sources + 1, 0);
+    int64_t samples = 0;
+    for (int s = 0; s < num_sources; ++s) {
+        const flr_source &q = sources[s];
+        if (q.num_pts < 1 || q.offset < 0 || q.projections_start_ind < 0 || q.tdi_start_ind < 0)
+            return fail(FLR_ERR_INVALID_ARGUMENT, "source " + std::to_string(s) + ": bad size or offset");
+        if (!std::isfinite(q.scale))
+            return fail(FLR_ERR_INVALID_ARGUMENT, "source " + std::to_string(s) + ": scale is not finite");
+        FusedSource &f = fs[s];
+        f.h = static_cast<const char *>(h_all) + q.offset * hsz;
+        f.out = static_cast<char *>(out_all) + 3 * q.offset * osz;
+        f.num_pts = q.num_pts;
+        f.proj_start = q.projections_start_ind;
+        f.tdi_start = q.tdi_start_ind;
+        f.t0 = q.t0;
+        f.scale = q.scale == 0.0 ? 1.0 : q.scale;
+        for (int c = 0; c < 3; ++c) {
+            f.k[c] = q.k[c];
+            f.u[c] = q.u[c];
+            f.v[c] = q.v[c];
+        }
+        const int64_t nt = (int64_t)ft[s] + tiles_for(q.num_pts, f.tdi_start, zero_edges,
+                                                      b->tile_len);
+        if (nt > INT32_MAX) return fail(FLR_ERR_UNSUPPORTED, "batch has too many tiles");
+        ft[s + 1] = (int)nt;
+        samples += q.num_pts;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    FLR_HIP(hipEventSynchronize(b->uploaded));  // the staging buffers are free again
+    std::memcpy(b->h_fused, fs.data(), sizeof(FusedSource) * num_sources);
+    std::memcpy(b->h_fused_tiles, ft.data(), sizeof(int) * (num_sources + 1));
+    FLR_HIP(hipMemcpyAsync(b->d_fused, b->h_fused, sizeof(FusedSource) * num_sources,
+                           hipMemcpyHostToDevice, st));
+    FLR_HIP(hipMemcpyAsync(b->d_fused_tiles, b->h_fused_tiles, sizeof(int) * (num_sources + 1),
+                           hipMemcpyHostToDevice, st));
+    FLR_HIP(hipEventRecord(b->uploaded, st));
+    b->num_sources = num_sources;
+    b->samples = samples;
+    b->fused_tiles = ft[num_sources];
+    return FLR_OK;
+}
+
+int flr_fused_create(flr_response *r, const flr_source *sources, int num_sources,
+                     const void *h_all, void *out_all, double dt, double sampling_frequency,
+                     int flags, flr_batch **out)
+{
+    if (!out) return fail(FLR_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    if (num_sources < 1 || !sources) return fail(FLR_ERR_INVALID_ARGUMENT, "no sources");
+    flr_batch *b = nullptr;
+    int st = flr_fused_reserve(r, num_sources, dt, sampling_frequency, flags, &b);
+    if (st) return st;
+    DeviceGuard g(r->device);
+    st = flr_fused_bind(b, sources, num_sources, h_all, out_all, nullptr);
+    if (st == FLR_OK && hipEventSynchronize(b->uploaded) != hipSuccess)
+        st = fail(FLR_ERR_HIP, "fused batch upload");
+    if (st) {
+        flr_batch_destroy(b);
+        return st;
+    }
+    *out = b;
+    return FLR_OK;
+}
+
 int flr_batch_run(flr_batch *b, void *stream)
 {
     if (!b || !b->r) return fail(FLR_ERR_INVALID_ARGUMENT, "batch is NULL");
@@ -822,18 +980,80 @@ int flr_batch_run(flr_batch *b, void *stream)
     hipStream_t s = (hipStream_t)stream;
     const bool exact = (b->flags & FLR_FLAG_EXACT) != 0;
     ProjConfig pc;
-    int st = make_proj_config(r, b->dt, b->fs, b->flags, pc);
-    if (st) return st;
     TdiConfig tc;
     StockTdi stock;
+    int st;
+    if (b->fused) {
+        if ((st = make_fused_config(r, b->dt, b->fs, b->flags, b->tile_passes, pc, tc, stock)))
+            return st;
+        Batch<FusedSource> bf{b->d_fused, b->d_fused_tiles, b->num_sources, b->fused_tiles};
+        hipError_t e = launch_fused_batch(pc, tc, bf, r->orbits->view(), r->lagrange(), exact,
+                                          (b->flags & FLR_FLAG_MIXED) != 0, b->tile_passes,
+                                          b->grid_per_cu, stock, s);
+        if (e != hipSuccess) return hip_fail(e, "fused batch launch");
+        return FLR_OK;
+    }
+    st = make_proj_config(r, b->dt, b->fs, b->flags, pc);
+    if (st) return st;
     st = make_tdi_config(r, b->dt, b->fs, b->flags, tc, stock);
     if (st) return st;
     Batch<ProjSource> bp{b->d_proj, b->d_proj_tiles, b->num_sources, b->proj_tiles};
     Batch<TdiSource> bt{b->d_tdi, b->d_tdi_tiles, b->num_sources, b->tdi_tiles};
-    hipError_t e = launch_projection_batch(pc, bp, r->orbits->view(), r->lagrange(), exact, s);
+    hipError_t e = launch_projection_batch(pc, bp, r->orbits->view(), r->lagrange(), exact,
+                                           b->grid_per_cu, s);
     if (e != hipSuccess) return hip_fail(e, "batch projection launch");
-    e = launch_tdi_batch(tc, bt, r->orbits->view(), r->lagrange(), exact, stock, s);
+    e = launch_tdi_batch(tc, bt, r->orbits->view(), r->lagrange(), exact, stock, b->grid_per_cu, s);
     if (e != hipSuccess) return hip_fail(e, "batch tdi launch");
+    return FLR_OK;
+}
+
+int flr_batch_set_launch(flr_batch *b, int tile_samples, int grid_per_cu)
+{
+    if (!b || !b->r) return fail(FLR_ERR_INVALID_ARGUMENT, "batch is NULL");
+    if (grid_per_cu < 0 || grid_per_cu > 64)
+        return fail(FLR_ERR_INVALID_ARGUMENT, "grid_per_cu must be in [1, 64] (0 keeps it)");
+    int passes = b->tile_passes;
+    if (tile_samples != 0) {
+        if (!b->fused) {
+            if (tile_samples != kBlock)
+                return fail(FLR_ERR_UNSUPPORTED, "unfused batches use 256-sample tiles");
+        } else {
+            passes = tile_samples / kBlock;
+            const bool mixed = (b->flags & FLR_FLAG_MIXED) != 0;
+            const bool ok = tile_samples % kBlock == 0 &&
+                            (mixed ? (passes == 1 || passes == 2 || passes == 4) : passes == kFusedPasses);
+            if (!ok)
+                return fail(FLR_ERR_UNSUPPORTED, mixed ? "mixed fused tiles are 256, 512 or 1024 samples"
+                                                       : "FP64 fused tiles are 512 samples");
+        }
+    }
+    DeviceGuard g(b->r->device);
+    if (b->fused && passes != b->tile_passes) {
+        ProjConfig pc;
+        TdiConfig tc;
+        StockTdi stock;
+        int st = make_fused_config(b->r, b->dt, b->fs, b->flags, passes, pc, tc, stock);
+        if (st) return st;
+        b->tile_passes = passes;
+        b->tile_len = tc.tile;
+        if (b->num_sources > 0) {  // re-tile the bound sources (host copy in the staging buffer)
+            FLR_HIP(hipEventSynchronize(b->uploaded));
+            const int zero_edges = (b->flags & FLR_FLAG_ZERO_EDGES) ? 1 : 0;
+            int *ft = b->h_fused_tiles;
+            ft[0] = 0;
+            for (int s = 0; s < b->num_sources; ++s) {
+                const FusedSource &f = b->h_fused[s];
+                const int64_t nt = (int64_t)ft[s] + tiles_for(f.num_pts, f.tdi_start, zero_edges,
+                                                              b->tile_len);
+                if (nt > INT32_MAX) return fail(FLR_ERR_UNSUPPORTED, "batch has too many tiles");
+                ft[s + 1] = (int)nt;
+            }
+            FLR_HIP(hipMemcpy(b->d_fused_tiles, ft, sizeof(int) * (b->num_sources + 1),
+                              hipMemcpyHostToDevice));
+            b->fused_tiles = ft[b->num_sources];
+        }
+    }
+    if (grid_per_cu != 0) b->grid_per_cu = grid_per_cu;
     return FLR_OK;
 }
 

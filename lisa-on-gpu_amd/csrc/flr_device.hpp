@@ -47,9 +47,12 @@ struct OrbitNode {
     double tau;       // t - dt*k
 };
 
+// EXACT: the window index from t / dt as the CPU path; otherwise from t * (1/dt), which can
+// pick the neighbouring window at a node -- the interpolant is continuous there.
+template <bool EXACT = true>
 __device__ __forceinline__ OrbitNode orbit_node(const OrbitView &o, double t)
 {
-    double q = t / o.dt;
+    double q = EXACT ? t / o.dt : t * o.inv_dt;
     int w;
     if (!(q >= 0.0))
         w = 0;
@@ -115,6 +118,19 @@ __device__ __forceinline__ double lagrange_A_fast(const LagrangeView &lg, double
     return __builtin_fma(a.y, frac, a.x);
 }
 
+// Explicit weight of tap x[d + r], r in [-(h-1), h], in the fast stencil at fraction e (the
+// weights the fast form applies implicitly): A B for x[d+1], A e for x[d], A D E_{j-1}/(j+e) for
+// x[d+1+j] and A D E_{j-1}/(j+1-e) for x[d-j].
+__device__ __forceinline__ double lagrange_weight(const LagrangeView &lg, double e, int r)
+{
+    const double A = lagrange_A_fast(lg, e);
+    if (r == 1) return A * (1.0 - e);
+    if (r == 0) return A * e;
+    const double D = e * (1.0 - e);
+    if (r > 1) return A * D * lg.E[r - 2] / ((double)(r - 1) + e);
+    return A * D * lg.E[-r - 1] / ((double)(1 - r) - e);
+}
+
 // ------------------------------------------------------------------ stencils
 // x points at sample d of a window; taps x[d-(h-1)] .. x[d+h] are read.
 //
@@ -171,7 +187,7 @@ __device__ __forceinline__ double stencil_fast(const double *x, double e, const 
         T = __builtin_fma(T, Q, num * (lg.E[j - 1] * P));
         P = P * Q;
     }
-    return A * __builtin_fma(D, T * (1.0 / P), __builtin_fma(B, x[1], e * x[0]));
+    return A * __builtin_fma(D, T * rcp_nr2(P), __builtin_fma(B, x[1], e * x[0]));
 }
 
 // Complex: the h+ and hx sums share each denominator's reciprocal.
@@ -230,7 +246,7 @@ __device__ __forceinline__ double2 stencil_fast_c(const double2 *x, double e, co
         Ti = __builtin_fma(Ti, Q, ni * EP);
         P = P * Q;
     }
-    double invP = 1.0 / P;
+    double invP = rcp_nr2(P);
     double2 x1 = x[1], x0 = x[0];
     double2 out;
     out.x = A * __builtin_fma(D, Tr * invP, __builtin_fma(B, x1.x, e * x0.x));

@@ -25,6 +25,7 @@ struct OrbitView {
     const double *table;  // [N][kOrbitStride]
     double dt;
     int N;
+    double inv_dt;        // 1 / dt (fast-form node lookup)
 };
 
 struct LagrangeView {
@@ -85,6 +86,7 @@ struct TdiConfig {
     int zero_edges;
     double s2, s6, s3;      // sqrt(2), sqrt(6), sqrt(3)
     int *err;
+    int tile;               // fused launches: output samples per tile
 };
 
 struct TdiSource {
@@ -122,26 +124,59 @@ struct Batch {
     int total_tiles;
 };
 
-// Tiles a source needs: one per kBlock samples of [start, n - start), or of [0, n) when the
+// Tiles a source needs: one per `tile` samples of [start, n - start), or of [0, n) when the
 // edges are zeroed too.
-inline int tiles_for(int num_delays, int start, int zero_edges)
+inline int tiles_for(int num_delays, int start, int zero_edges, int tile = kBlock)
 {
     const int span = zero_edges ? num_delays : num_delays - 2 * start;
-    return span > 0 ? (span + kBlock - 1) / kBlock : 0;
+    return span > 0 ? (span + tile - 1) / tile : 0;
 }
+
+// ------------------------------------------------------------------ fused response
+// Projection and TDI in one launch: a tile of output samples projects the six links over its
+// TDI window into LDS and runs TDI from there, so y never goes to HBM.
+constexpr int kFusedPasses = 2;  // default tile: kFusedPasses * kBlock output samples
+
+struct FusedSource {
+    const void *h;          // [num_pts] complex: double2 (FP64), or float2 holding h / scale
+    void *out;              // [3][num_pts] double (FP64) or float (mixed)
+    int num_pts;
+    int proj_start;         // projections_start_ind: y is zero outside [proj_start, n - proj_start)
+    int tdi_start;          // tdi_start_ind
+    int pad;
+    double t0;
+    double scale;           // outputs are scale * (response of the stored h)
+    double k[3], u[3], v[3];
+};
 
 // Launchers (flr_kernels.hip).
 hipError_t launch_projection(const ProjConfig &cfg, const ProjSource &src, const OrbitView &orb,
                              const LagrangeView &lg, bool exact, hipStream_t stream);
 hipError_t launch_projection_batch(const ProjConfig &cfg, const Batch<ProjSource> &batch,
                                    const OrbitView &orb, const LagrangeView &lg, bool exact,
-                                   hipStream_t stream);
+                                   int grid_per_cu, hipStream_t stream);
 hipError_t launch_tdi(const TdiConfig &cfg, const TdiSource &src, const OrbitView &orb,
                       const LagrangeView &lg, bool exact, const StockTdi &stock,
                       hipStream_t stream);
 hipError_t launch_tdi_batch(const TdiConfig &cfg, const Batch<TdiSource> &batch,
                             const OrbitView &orb, const LagrangeView &lg, bool exact,
-                            const StockTdi &stock, hipStream_t stream);
+                            const StockTdi &stock, int grid_per_cu, hipStream_t stream);
+// Fused batch: exact (FP64 only) or fast stencils; mixed = float2 h in, float out. Tiles of
+// tile_passes * kBlock output samples (1, 2 or 4 for mixed, 2 for FP64); grid_per_cu
+// persistent workgroups per CU.
+hipError_t launch_fused_batch(const ProjConfig &pc, const TdiConfig &tc,
+                              const Batch<FusedSource> &batch, const OrbitView &orb,
+                              const LagrangeView &lg, bool exact, bool mixed, int tile_passes,
+                              int grid_per_cu, const StockTdi &stock, hipStream_t stream);
+hipError_t launch_fused_fp64(const ProjConfig &pc, const TdiConfig &tc,
+                             const Batch<FusedSource> &batch, const OrbitView &orb,
+                             const LagrangeView &lg, bool exact, int grid_per_cu,
+                             const StockTdi &stock, hipStream_t stream);
+hipError_t launch_fused_mixed(const ProjConfig &pc, const TdiConfig &tc,
+                              const Batch<FusedSource> &batch, const OrbitView &orb,
+                              const LagrangeView &lg, int tile_passes, int grid_per_cu,
+                              const StockTdi &stock, hipStream_t stream);
+size_t fused_lds_bytes(const ProjConfig &pc, const TdiConfig &tc, bool generic);
 hipError_t launch_orbit_eval(const OrbitView &orb, const double *t, int n, int row, int sc,
                              double *ltt_out, double *pos_out, hipStream_t stream);
 hipError_t launch_clear(int *word, hipStream_t stream);

@@ -17,27 +17,9 @@
 // Each tile function is launched two ways: one workgroup per tile of a single source, or a
 // persistent batch kernel whose workgroups walk contiguous runs of a flattened (source, tile)
 // list -- many sources of uneven length in one launch.
-#include "flr_device.hpp"
-#include "flr_tdi_stock.hpp"
+#include "flr_paths.hpp"
 
 namespace flr {
-
-// Sample range of tile `tile` of a source: lanes i in [i0, i0 + kBlock), computed samples
-// [lo_valid, hi_valid), and the tile's first/last computed sample.
-struct TileRange {
-    int i0, lo_valid, hi_valid, first, last;
-};
-
-__device__ __forceinline__ TileRange tile_range(int tile, int num_delays, int start, int zero_edges)
-{
-    TileRange r;
-    r.i0 = (zero_edges ? 0 : start) + tile * kBlock;
-    r.lo_valid = start;
-    r.hi_valid = num_delays - start;
-    r.first = max(r.i0, r.lo_valid);
-    r.last = min(r.i0 + kBlock, r.hi_valid) - 1;
-    return r;
-}
 
 // ------------------------------------------------------------------ projection
 
@@ -46,7 +28,6 @@ __device__ __forceinline__ void projection_tile(const ProjConfig &cfg, const Pro
                                                 int tile, const OrbitView &orb,
                                                 const LagrangeView &lg, double2 *hwin)
 {
-    const int h = H > 0 ? H : lg.h;
     const int tid = threadIdx.x;
     const TileRange tr = tile_range(tile, src.num_delays, src.start, cfg.zero_edges);
     const int i = tr.i0 + tid;
@@ -59,6 +40,8 @@ __device__ __forceinline__ void projection_tile(const ProjConfig &cfg, const Pro
     }
     if (tr.first > tr.last) return;  // whole tile outside the computed range
 
+    // The window is sized from the orbit extent, so nothing has to be known about the delays
+    // before it is staged.
     const int win_lo = tr.first - cfg.halo_lo;
     const int win_len = tr.last + cfg.halo_hi - win_lo + 1;
     if (win_len > cfg.win_cap || win_lo < 0 || win_lo + win_len > src.num_inputs) {
@@ -73,70 +56,15 @@ __device__ __forceinline__ void projection_tile(const ProjConfig &cfg, const Pro
     const int ic = valid ? i : tr.first;
     const double t = src.t_data ? src.t_data[ic] : src.t0 + (double)ic * cfg.dt;
     const double t0_offset = src.t_data ? src.t_data[0] : src.t0 + (double)0 * cfg.dt;
-    const OrbitNode nr = orbit_node(orb, t);
     __syncthreads();
     if (!valid) return;
     const double y_deps = EXACT ? rcp_nr2(lg.deps) : 0.0;
-
-    // Per receiving spacecraft: its stencil (shared by its two links), then each link's
-    // emitter geometry and stencil; the window is sized from the orbit extent, so nothing has
-    // to be known about the delays before it is staged. Operation order per link follows
-    // LISAResponse.cu:559-668.
-    bool bad = false;
-#pragma unroll
-    for (int s = 0; s < 3; ++s) {
-        double xr[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) xr[c] = orbit_val(nr, 6 + 3 * s + c);
-        int d_rec;
-        double e_rec;
-        split_delay(t - dot3(src.k, xr) * kCInv, t0_offset, cfg.fs, d_rec, e_rec);
-        const int lr = d_rec - win_lo;
-        const bool br = (lr - (h - 1) < 0) || (lr + h >= win_len);
-        double2 hr = make_double2(0.0, 0.0);
-        if (!br) hr = stencil_c<EXACT, H>(hwin + lr, e_rec, lg, y_deps);
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int row = cfg.rec_row[s][q];
-            const int se = cfg.em_sc[s][q];
-            const double L = orbit_val(nr, row);
-            const double t_em = t - L;
-            const OrbitNode ne = orbit_node(orb, t_em);
-            double xe[3], n[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) xe[c] = orbit_val(ne, 6 + 3 * se + c);
-            double norm = 0.0;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                double nt = xr[c] - xe[c];
-                n[c] = nt;
-                norm += nt * nt;
-            }
-            norm = sqrt(norm);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) n[c] = n[c] / norm;
-            const double un = dot3(src.u, n);
-            const double vn = dot3(src.v, n);
-            const double xi_p = 0.5 * ((un * un) - (vn * vn));
-            const double xi_c = un * vn;
-            const double kn = dot3(src.k, n);
-            int d_em;
-            double e_em;
-            split_delay(t_em - dot3(src.k, xe) * kCInv, t0_offset, cfg.fs, d_em, e_em);
-            const int le = d_em - win_lo;
-            const bool bq = br || (le - (h - 1) < 0) || (le + h >= win_len);
-            double out = __builtin_nan("");
-            if (!bq) {
-                const double2 he = stencil_c<EXACT, H>(hwin + le, e_em, lg, y_deps);
-                const double pre = 1. / (1. - kn);
-                const double large = (he.x - hr.x) * xi_p + (he.y - hr.y) * xi_c;
-                out = pre * large;
-            }
-            bad |= bq;
-            src.y[(size_t)row * src.num_delays + i] = out;
-        }
-    }
-    if (bad) atomicOr(cfg.err, kErrWindow);
+    double *y = src.y + i;
+    const size_t stride = (size_t)src.num_delays;
+    if (project_links<EXACT, H>(cfg, src.k, src.u, src.v, t, t0_offset, orb, lg, hwin, win_lo,
+                                win_len, y_deps,
+                                [&](int row, double val) { y[row * stride] = val; }))
+        atomicOr(cfg.err, kErrWindow);
 }
 
 // ------------------------------------------------------------------ TDI (any table)
@@ -189,7 +117,6 @@ __device__ __forceinline__ void tdi_tile(const TdiConfig &cfg, const TdiSource &
                                          const OrbitView &orb, const LagrangeView &lg,
                                          double *smem)
 {
-    const int h = H > 0 ? H : lg.h;
     double *ywin = smem;                        // [6][win_cap]
     double *Ls = smem + kNLinks * cfg.win_cap;  // [6][kBlock]
     const int tid = threadIdx.x;
@@ -203,7 +130,7 @@ __device__ __forceinline__ void tdi_tile(const TdiConfig &cfg, const TdiSource &
     const double t = src.t_arr ? src.t_arr[ic] : src.t0 + (double)ic * cfg.dt;
     const double t0_offset = src.t_arr ? src.t_arr[0] : src.t0 + (double)0 * cfg.dt;
     {
-        const OrbitNode nd = orbit_node(orb, t);
+        const OrbitNode nd = orbit_node<EXACT>(orb, t);
 #pragma unroll
         for (int r = 0; r < kNLinks; ++r) Ls[r * kBlock + tid] = orbit_val(nd, r);
     }
@@ -216,34 +143,9 @@ __device__ __forceinline__ void tdi_tile(const TdiConfig &cfg, const TdiSource &
     for (int c = 0; c < 3; ++c)
         acc[c] = cfg.overwrite ? 0.0 : src.out[(size_t)c * src.num_delays + i];
 
-    bool bad = false;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        for (int u = cfg.ch_begin[c]; u < cfg.ch_begin[c + 1]; ++u) {
-            const TdiUnit un = cfg.units[u];
-            // Same-t delay chain: every light travel time is taken at the undelayed t and
-            // subtracted in chain order (LISAResponse.cu:314-333).
-            double delay = t;
-            for (int s = 0; s < un.chain_len; ++s) {
-                const int row = cfg.chain_rows[un.chain_start + s];
-                if (row >= 0) delay -= Ls[row * kBlock + tid];
-            }
-            int d;
-            double e;
-            split_delay(delay, t0_offset, cfg.fs, d, e);
-            const int ld = d - win_lo;
-            double val;
-            if ((ld - (h - 1) >= 0) && (ld + h < win_len)) {
-                val = stencil<EXACT, H>(ywin + un.base_row * cfg.win_cap + ld, e, lg, y_deps);
-            } else {
-                bad = true;
-                val = __builtin_nan("");
-            }
-            val *= un.sign;
-            acc[c] += val;
-        }
-    }
-    if (bad) atomicOr(cfg.err, kErrWindow);
+    if (tdi_units<EXACT, H>(cfg, t, t0_offset, Ls + tid, kBlock, ywin, cfg.win_cap, win_lo, win_len,
+                            lg, y_deps, acc))
+        atomicOr(cfg.err, kErrWindow);
     tdi_store(cfg, src, i, acc);
 }
 
@@ -269,7 +171,7 @@ __device__ __forceinline__ void tdi_equal_arm_tile(const TdiConfig &cfg, const T
     const int ic = valid ? i : tr.first;
     const double t = src.t_arr ? src.t_arr[ic] : src.t0 + (double)ic * cfg.dt;
     const double t0_offset = src.t_arr ? src.t_arr[0] : src.t0 + (double)0 * cfg.dt;
-    const double L = orbit_val(orbit_node(orb, t), rm.row[0]);
+    const double L = orbit_val(orbit_node<EXACT>(orb, t), rm.row[0]);
     int rowoff[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) rowoff[k] = rm.row[k] * cfg.win_cap - win_lo;
@@ -288,37 +190,6 @@ __device__ __forceinline__ void tdi_equal_arm_tile(const TdiConfig &cfg, const T
 }
 
 // ------------------------------------------------------------------ kernels
-
-// Source of flattened tile g: tile_start is a prefix sum over sources (uniform search).
-template <typename Source>
-__device__ __forceinline__ int batch_source(const Batch<Source> &b, int g)
-{
-    int lo = 0, hi = b.num_sources;  // tile_start[lo] <= g < tile_start[hi]
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (b.tile_start[mid] <= g)
-            lo = mid;
-        else
-            hi = mid;
-    }
-    return lo;
-}
-
-// Workgroup b of G walks the contiguous tiles [b*T/G, (b+1)*T/G): neighbouring tiles share
-// halo samples, which then stay in the same XCD's L2.
-#define FLR_BATCH_LOOP(batch, body)                                                        \
-    do {                                                                                   \
-        const long long T_ = (batch).total_tiles;                                          \
-        const int g0_ = (int)((T_ * blockIdx.x) / gridDim.x);                              \
-        const int g1_ = (int)((T_ * (blockIdx.x + 1)) / gridDim.x);                        \
-        for (int g = g0_; g < g1_; ++g) {                                                  \
-            __syncthreads(); /* the previous tile's LDS readers are done */                \
-            const int s_ = batch_source(batch, g);                                         \
-            const auto src = (batch).sources[s_];                                          \
-            const int tile = g - (batch).tile_start[s_];                                   \
-            body;                                                                          \
-        }                                                                                  \
-    } while (0)
 
 template <bool EXACT, int H>
 __global__ __launch_bounds__(kBlock) void projection_kernel(ProjConfig cfg, ProjSource src,
@@ -393,16 +264,7 @@ __global__ void clear_kernel(int *word)
 
 // ------------------------------------------------------------------ launchers
 
-template <typename Kernel>
-static hipError_t set_lds_limit(Kernel k, size_t bytes)
-{
-    if (bytes <= 64 * 1024) return hipSuccess;
-    return hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-}
-
-// Persistent batch grid: enough workgroups to fill every CU a few times over, never more than
-// there are tiles.
-static int batch_grid(int total_tiles)
+int batch_grid(int total_tiles, int grid_per_cu)
 {
     static int cus = 0;
     if (cus == 0) {
@@ -412,18 +274,8 @@ static int batch_grid(int total_tiles)
             cus <= 0)
             cus = 256;
     }
-    const int grid = cus * 8;
+    const int grid = cus * (grid_per_cu > 0 ? grid_per_cu : 8);
     return total_tiles < grid ? total_tiles : grid;
-}
-
-template <typename Kernel, typename... Args>
-static hipError_t launch(Kernel k, int nblocks, size_t shmem, hipStream_t stream, Args... args)
-{
-    if (nblocks <= 0) return hipSuccess;
-    hipError_t err = set_lds_limit(k, shmem);
-    if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(k, dim3(nblocks), dim3(kBlock), shmem, stream, args...);
-    return hipGetLastError();
 }
 
 static size_t proj_lds(const ProjConfig &cfg) { return (size_t)cfg.win_cap * sizeof(double2); }
@@ -442,20 +294,15 @@ hipError_t launch_projection(const ProjConfig &cfg, const ProjSource &src, const
 
 hipError_t launch_projection_batch(const ProjConfig &cfg, const Batch<ProjSource> &b,
                                    const OrbitView &orb, const LagrangeView &lg, bool exact,
-                                   hipStream_t stream)
+                                   int grid_per_cu, hipStream_t stream)
 {
-    const int nb = batch_grid(b.total_tiles);
+    const int nb = batch_grid(b.total_tiles, grid_per_cu);
     const size_t shm = proj_lds(cfg);
     if (lg.h == 13)
         return exact ? launch(projection_batch_kernel<true, 13>, nb, shm, stream, cfg, b, orb, lg)
                      : launch(projection_batch_kernel<false, 13>, nb, shm, stream, cfg, b, orb, lg);
     return exact ? launch(projection_batch_kernel<true, 0>, nb, shm, stream, cfg, b, orb, lg)
                  : launch(projection_batch_kernel<false, 0>, nb, shm, stream, cfg, b, orb, lg);
-}
-
-static bool use_equal_arm(const StockTdi &st, const LagrangeView &lg)
-{
-    return st.generation != 0 && st.equal_arm && lg.h == 13;
 }
 
 static size_t tdi_lds(const TdiConfig &cfg, bool equal_arm)
@@ -486,9 +333,9 @@ hipError_t launch_tdi(const TdiConfig &cfg, const TdiSource &src, const OrbitVie
 
 hipError_t launch_tdi_batch(const TdiConfig &cfg, const Batch<TdiSource> &b, const OrbitView &orb,
                             const LagrangeView &lg, bool exact, const StockTdi &st,
-                            hipStream_t stream)
+                            int grid_per_cu, hipStream_t stream)
 {
-    const int nb = batch_grid(b.total_tiles);
+    const int nb = batch_grid(b.total_tiles, grid_per_cu);
     if (use_equal_arm(st, lg)) {
         const size_t shm = tdi_lds(cfg, true);
         const RowMap &rm = st.rows;
@@ -519,6 +366,24 @@ hipError_t launch_clear(int *word, hipStream_t stream)
 {
     hipLaunchKernelGGL(clear_kernel, dim3(1), dim3(64), 0, stream, word);
     return hipGetLastError();
+}
+
+size_t fused_lds_bytes(const ProjConfig &pc, const TdiConfig &tc, bool generic)
+{
+    return (size_t)pc.win_cap * sizeof(double2) +
+           ((size_t)kNLinks * tc.win_cap + (generic ? (size_t)kNLinks * kBlock : (size_t)2 * kFir)) *
+               sizeof(double);
+}
+
+hipError_t launch_fused_batch(const ProjConfig &pc, const TdiConfig &tc,
+                              const Batch<FusedSource> &b, const OrbitView &orb,
+                              const LagrangeView &lg, bool exact, bool mixed, int tile_passes,
+                              int grid_per_cu, const StockTdi &st, hipStream_t stream)
+{
+    if (lg.h != 13) return hipErrorInvalidValue;  // fused kernels are built for order 25
+    if (mixed) return launch_fused_mixed(pc, tc, b, orb, lg, tile_passes, grid_per_cu, st, stream);
+    if (tile_passes != 2) return hipErrorInvalidValue;
+    return launch_fused_fp64(pc, tc, b, orb, lg, exact, grid_per_cu, st, stream);
 }
 
 }  // namespace flr

@@ -1,0 +1,212 @@
+// flr_paths.hpp -- per-sample paths and launch plumbing shared by the kernel translation units:
+// the tile range of a source, the six-link projection of one sample, the unit-by-unit TDI of
+// one sample, the persistent (source, tile) loop and the launch helpers.
+#pragma once
+
+#include "flr_device.hpp"
+#include "flr_tdi_stock.hpp"
+
+#include <type_traits>
+
+namespace flr {
+
+// Sample range of tile `tile` of a source: lanes i in [i0, i0 + kBlock), computed samples
+// [lo_valid, hi_valid), and the tile's first/last computed sample.
+struct TileRange {
+    int i0, lo_valid, hi_valid, first, last;
+};
+
+__device__ __forceinline__ TileRange tile_range(int tile, int num_delays, int start, int zero_edges)
+{
+    TileRange r;
+    r.i0 = (zero_edges ? 0 : start) + tile * kBlock;
+    r.lo_valid = start;
+    r.hi_valid = num_delays - start;
+    r.first = max(r.i0, r.lo_valid);
+    r.last = min(r.i0 + kBlock, r.hi_valid) - 1;
+    return r;
+}
+
+// ------------------------------------------------------------------ projection
+
+// Projections of all six links at time t from the staged h window (hwin[0] = sample win_lo):
+// per receiving spacecraft its stencil (shared by its two links), then each link's emitter
+// geometry and stencil; store(row, value) receives y of table row `row`. Operation order per
+// link follows LISAResponse.cu:559-668. Returns true if a stencil fell outside the window (its
+// links get NaN).
+template <bool EXACT, int H, typename Store>
+__device__ __forceinline__ bool project_links(const ProjConfig &cfg, const double (&k)[3],
+                                              const double (&u)[3], const double (&v)[3],
+                                              double t, double t0_offset, const OrbitView &orb,
+                                              const LagrangeView &lg, const double2 *hwin,
+                                              int win_lo, int win_len, double y_deps, Store &&store)
+{
+    const int h = H > 0 ? H : lg.h;
+    const OrbitNode nr = orbit_node<EXACT>(orb, t);
+    bool bad = false;
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        double xr[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) xr[c] = orbit_val(nr, 6 + 3 * s + c);
+        int d_rec;
+        double e_rec;
+        split_delay(t - dot3(k, xr) * kCInv, t0_offset, cfg.fs, d_rec, e_rec);
+        const int lr = d_rec - win_lo;
+        const bool br = (lr - (h - 1) < 0) || (lr + h >= win_len);
+        double2 hr = make_double2(0.0, 0.0);
+        if (!br) hr = stencil_c<EXACT, H>(hwin + lr, e_rec, lg, y_deps);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int row = cfg.rec_row[s][q];
+            const int se = cfg.em_sc[s][q];
+            const double L = orbit_val(nr, row);
+            const double t_em = t - L;
+            const OrbitNode ne = orbit_node<EXACT>(orb, t_em);
+            double xe[3], n[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) xe[c] = orbit_val(ne, 6 + 3 * se + c);
+            double norm = 0.0;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                double nt = xr[c] - xe[c];
+                n[c] = nt;
+                norm += nt * nt;
+            }
+            if constexpr (EXACT) {
+                norm = sqrt(norm);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) n[c] = n[c] / norm;
+            } else {  // one reciprocal square root instead of a root and three divisions
+                const double inv = rsqrt(norm);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) n[c] = n[c] * inv;
+            }
+            const double un = dot3(u, n);
+            const double vn = dot3(v, n);
+            const double xi_p = 0.5 * ((un * un) - (vn * vn));
+            const double xi_c = un * vn;
+            const double kn = dot3(k, n);
+            int d_em;
+            double e_em;
+            split_delay(t_em - dot3(k, xe) * kCInv, t0_offset, cfg.fs, d_em, e_em);
+            const int le = d_em - win_lo;
+            const bool bq = br || (le - (h - 1) < 0) || (le + h >= win_len);
+            double out = __builtin_nan("");
+            if (!bq) {
+                const double2 he = stencil_c<EXACT, H>(hwin + le, e_em, lg, y_deps);
+                const double pre = EXACT ? 1. / (1. - kn) : rcp_nr2(1. - kn);
+                const double large = (he.x - hr.x) * xi_p + (he.y - hr.y) * xi_c;
+                out = pre * large;
+            }
+            bad |= bq;
+            store(row, out);
+        }
+    }
+    return bad;
+}
+
+// ------------------------------------------------------------------ TDI
+
+// Every unit of every channel at time t, summed per channel in index order into acc. Ls[r *
+// ls_stride] is the light travel time of table row r at t; row r of the y window starts at
+// ywin + r * cap. Returns true if a stencil fell outside the window (its channel gets NaN).
+template <bool EXACT, int H>
+__device__ __forceinline__ bool tdi_units(const TdiConfig &cfg, double t, double t0_offset,
+                                          const double *Ls, int ls_stride, const double *ywin,
+                                          int cap, int win_lo, int win_len, const LagrangeView &lg,
+                                          double y_deps, double (&acc)[3])
+{
+    const int h = H > 0 ? H : lg.h;
+    bool bad = false;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        for (int u = cfg.ch_begin[c]; u < cfg.ch_begin[c + 1]; ++u) {
+            const TdiUnit un = cfg.units[u];
+            // Same-t delay chain: every light travel time is taken at the undelayed t and
+            // subtracted in chain order (LISAResponse.cu:314-333).
+            double delay = t;
+            for (int s = 0; s < un.chain_len; ++s) {
+                const int row = cfg.chain_rows[un.chain_start + s];
+                if (row >= 0) delay -= Ls[row * ls_stride];
+            }
+            int d;
+            double e;
+            split_delay(delay, t0_offset, cfg.fs, d, e);
+            const int ld = d - win_lo;
+            double val;
+            if ((ld - (h - 1) >= 0) && (ld + h < win_len)) {
+                val = stencil<EXACT, H>(ywin + un.base_row * cap + ld, e, lg, y_deps);
+            } else {
+                bad = true;
+                val = __builtin_nan("");
+            }
+            val *= un.sign;
+            acc[c] += val;
+        }
+    }
+    return bad;
+}
+
+// ------------------------------------------------------------------ persistent loop
+
+// Source of flattened tile g: tile_start is a prefix sum over sources (uniform search).
+template <typename Source>
+__device__ __forceinline__ int batch_source(const Batch<Source> &b, int g)
+{
+    int lo = 0, hi = b.num_sources;  // tile_start[lo] <= g < tile_start[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (b.tile_start[mid] <= g)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Workgroup b of G walks the contiguous tiles [b*T/G, (b+1)*T/G): neighbouring tiles share
+// halo samples, which then stay in the same XCD's L2.
+#define FLR_BATCH_LOOP(batch, body)                                                        \
+    do {                                                                                   \
+        const long long T_ = (batch).total_tiles;                                          \
+        const int g0_ = (int)((T_ * blockIdx.x) / gridDim.x);                              \
+        const int g1_ = (int)((T_ * (blockIdx.x + 1)) / gridDim.x);                        \
+        for (int g = g0_; g < g1_; ++g) {                                                  \
+            __syncthreads(); /* the previous tile's LDS readers are done */                \
+            const int s_ = batch_source(batch, g);                                         \
+            const auto src = (batch).sources[s_];                                          \
+            const int tile = g - (batch).tile_start[s_];                                   \
+            body;                                                                          \
+        }                                                                                  \
+    } while (0)
+
+// ------------------------------------------------------------------ launch helpers
+
+template <typename Kernel>
+inline hipError_t set_lds_limit(Kernel k, size_t bytes)
+{
+    if (bytes <= 64 * 1024) return hipSuccess;
+    return hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+// Persistent batch grid: grid_per_cu workgroups per CU (enough to fill every CU a few times
+// over), never more than there are tiles (flr_kernels.hip).
+int batch_grid(int total_tiles, int grid_per_cu = 8);
+
+template <typename Kernel, typename... Args>
+inline hipError_t launch(Kernel k, int nblocks, size_t shmem, hipStream_t stream, Args... args)
+{
+    if (nblocks <= 0) return hipSuccess;
+    hipError_t err = set_lds_limit(k, shmem);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(k, dim3(nblocks), dim3(kBlock), shmem, stream, args...);
+    return hipGetLastError();
+}
+
+inline bool use_equal_arm(const StockTdi &st, const LagrangeView &lg)
+{
+    return st.generation != 0 && st.equal_arm && lg.h == 13;
+}
+
+}  // namespace flr

@@ -58,6 +58,37 @@ constexpr int base_of(int c, int x) { return canon(cyc(kX[x].link, c)); }
 constexpr int chain_of(int c, int x, int s) { return canon(cyc(kX[x].chain[s], c)); }
 constexpr int max_len(int nx) { return nx == 8 ? 3 : 7; }
 
+// The two X terms of chain length m combine to +-(y_la - y_lb): +1 or -1, or 0 if they are not
+// that pair.
+constexpr int pair_sign(int m, int la, int lb)
+{
+    int sa = 0, sb = 0, n = 0;
+    for (int x = 0; x < 16; ++x) {
+        if (kX[x].len != m) continue;
+        ++n;
+        if (kX[x].link == la)
+            sa += kX[x].sign;
+        else if (kX[x].link == lb)
+            sb += kX[x].sign;
+        else
+            return 0;
+    }
+    if (n != 2) return 0;
+    return (sa == 1 && sb == -1) ? 1 : (sa == -1 && sb == 1) ? -1 : 0;
+}
+
+// With equal arms all units of chain length m share one delay, and (by linearity of the
+// stencil) their sum is one stencil of a combined series: kind 0 = y13 - y12, kind 1 =
+// y31 - y21 (cyclically shifted for Y and Z), times group_sign.
+constexpr int group_kind(int m) { return pair_sign(m, 13, 12) != 0 ? 0 : 1; }
+constexpr int group_sign(int m)
+{
+    return pair_sign(m, 13, 12) != 0 ? pair_sign(m, 13, 12) : pair_sign(m, 31, 21);
+}
+static_assert(group_sign(0) && group_sign(1) && group_sign(2) && group_sign(3) &&
+                  group_sign(4) && group_sign(5) && group_sign(6) && group_sign(7),
+              "every chain length pairs two links of one kind");
+
 
 }  // namespace stock
 
@@ -78,6 +109,7 @@ __device__ __forceinline__ void static_for(F &&f)
 // from sinking the arithmetic producing it past later loops (which would keep every group's
 // intermediates alive at once).
 __device__ __forceinline__ void pin(double &x) { __asm__ volatile("" : "+v"(x)); }
+__device__ __forceinline__ void pin(int &x) { __asm__ volatile("" : "+v"(x)); }
 
 // Number of leading X terms x = 0, 1, ... whose chain length is at most m: once the groups of
 // chain lengths 0..m are evaluated, these units can be added to their channel in index order.
@@ -119,12 +151,21 @@ __device__ __forceinline__ bool tdi_equal_arm(double t, double L, double t0_offs
         static_for<3>([&](auto c) {
             int k = 0;
             static_for<NX>([&](auto x) {
-                if constexpr (stock::kX[x].len == m) xr[c][k++] = ywin + rowoff[stock::base_of(c, x)] + d;
+                // based at the lowest tap x[d - (H-1)], so every tap is a non-negative immediate
+                // offset; the pinned index keeps the compiler from choosing another base
+                if constexpr (stock::kX[x].len == m) {
+                    int off = rowoff[stock::base_of(c, x)] + d - (H - 1);
+                    pin(off);
+                    xr[c][k++] = ywin + off;
+                }
             });
         });
         double S[3][2] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
         double P = 1.0;
-#pragma unroll 1
+        // j fully unrolled: every tap address is the unit's base plus an immediate offset (the
+        // up/down pair of a unit loads with one ds_read2_b64); the memory barrier and the pins
+        // at the end of each j keep its loads and arithmetic together.
+#pragma unroll
         for (int j = 1; j < H; ++j) {
             const double E = lg.E[j - 1];
             if constexpr (EXACT) {
@@ -136,7 +177,8 @@ __device__ __forceinline__ bool tdi_equal_arm(double t, double L, double t0_offs
                 for (int c = 0; c < 3; ++c)
 #pragma unroll
                     for (int k = 0; k < 2; ++k)
-                        S[c][k] += E * (div_rcp(xr[c][k][1 + j], F, yF) + div_rcp(xr[c][k][-j], G, yG));
+                        S[c][k] += E * (div_rcp(xr[c][k][H + j], F, yF) +
+                                        div_rcp(xr[c][k][H - 1 - j], G, yG));
             } else {
                 const double qu = (double)j + e;
                 const double qd = (double)(j + 1) - e;
@@ -146,11 +188,18 @@ __device__ __forceinline__ bool tdi_equal_arm(double t, double L, double t0_offs
                 for (int c = 0; c < 3; ++c)
 #pragma unroll
                     for (int k = 0; k < 2; ++k) {
-                        const double num = __builtin_fma(xr[c][k][1 + j], qd, xr[c][k][-j] * qu);
+                        const double num = __builtin_fma(xr[c][k][H + j], qd, xr[c][k][H - 1 - j] * qu);
                         S[c][k] = __builtin_fma(S[c][k], Q, num * EP);
                     }
                 P = P * Q;
+                pin(P);
             }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                pin(S[c][0]);
+                pin(S[c][1]);
+            }
+            __asm__ volatile("" ::: "memory");
         }
         double A, B, C, D, invP = 0.0;
         if constexpr (EXACT) {
@@ -169,7 +218,7 @@ __device__ __forceinline__ bool tdi_equal_arm(double t, double L, double t0_offs
             int k = 0;
             static_for<NX>([&](auto x) {
                 if constexpr (stock::kX[x].len == m) {
-                    const double *p = xr[c][k];
+                    const double *p = xr[c][k] + (H - 1);  // p[0] = x[d], p[1] = x[d+1]
                     double val;
                     if constexpr (EXACT)
                         val = A * (B * p[1] + C * p[0] + D * S[c][k]);
@@ -193,6 +242,206 @@ __device__ __forceinline__ bool tdi_equal_arm(double t, double L, double t0_offs
         });
     });
     return ok;
+}
+
+// Column of the combined series (in place of the six y rows, same stride `cap`): row
+// 2*o + kind holds output o's combination of kind `kind` -- X, Y, Z themselves, or A, E, T
+// formed from them when aet is set. yc points at the column; rm maps canonical links to rows.
+__device__ __forceinline__ void linear_series(double *yc, int cap, const RowMap &rm, bool aet,
+                                              double s2, double s6, double s3)
+{
+    double y[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) y[k] = yc[rm.row[k] * cap];
+    double kd[2][3];
+    static_for<3>([&](auto c) {
+        kd[0][c] = y[stock::canon(stock::cyc(13, c))] - y[stock::canon(stock::cyc(12, c))];
+        kd[1][c] = y[stock::canon(stock::cyc(31, c))] - y[stock::canon(stock::cyc(21, c))];
+    });
+#pragma unroll
+    for (int kind = 0; kind < 2; ++kind) {
+        const double X = kd[kind][0], Y = kd[kind][1], Z = kd[kind][2];
+        if (aet) {
+            yc[(0 + kind) * cap] = (Z - X) / s2;
+            yc[(2 + kind) * cap] = (X - 2.0 * Y + Z) / s6;
+            yc[(4 + kind) * cap] = (X + Y + Z) / s3;
+        } else {
+            yc[(0 + kind) * cap] = X;
+            yc[(2 + kind) * cap] = Y;
+            yc[(4 + kind) * cap] = Z;
+        }
+    }
+}
+
+// Equal-arm stock TDI from the combined series (fast stencils): per chain length m one delay
+// t - m*L, one set of stencil weights and one stencil per output -- 3 stencils per group
+// instead of 6 units. Equal to the unit-by-unit sum up to rounding. Returns false if a stencil
+// fell outside the window (the outputs are NaN).
+template <int NX>
+__device__ __forceinline__ bool tdi_linear(double t, double L, double t0_offset, double fs,
+                                           const double *ywin, int cap, int win_lo, int win_len,
+                                           const LagrangeView &lg, double (&acc)[3])
+{
+    constexpr int H = 13;
+    constexpr int NG = stock::max_len(NX) + 1;
+    bool ok = true;
+    double Dm = t;
+    static_for<NG>([&](auto m) {
+        if constexpr (m > 0) Dm -= L;
+        int d;
+        double e;
+        split_delay(Dm, t0_offset, fs, d, e);
+        const bool gbad = (d - win_lo - (H - 1) < 0) || (d - win_lo + H >= win_len);
+        ok = ok && !gbad;
+        if (gbad) d = win_lo + H - 1;  // in-window stand-in; the result is discarded
+        constexpr int kind = stock::group_kind(m);
+        const double *xr[3];
+#pragma unroll
+        for (int o = 0; o < 3; ++o) {
+            int off = (2 * o + kind) * cap + d - win_lo - (H - 1);
+            pin(off);
+            xr[o] = ywin + off;  // lowest tap; every tap at a non-negative immediate offset
+        }
+        double S[3] = {0.0, 0.0, 0.0};
+        double P = 1.0;
+#pragma unroll
+        for (int j = 1; j < H; ++j) {
+            const double qu = (double)j + e;
+            const double qd = (double)(j + 1) - e;
+            const double Q = (double)(j * (j + 1)) + e * (1.0 - e);
+            const double EP = lg.E[j - 1] * P;
+#pragma unroll
+            for (int o = 0; o < 3; ++o) {
+                const double num = __builtin_fma(xr[o][H + j], qd, xr[o][H - 1 - j] * qu);
+                S[o] = __builtin_fma(S[o], Q, num * EP);
+                pin(S[o]);
+            }
+            P = P * Q;
+            pin(P);
+            __asm__ volatile("" ::: "memory");
+        }
+        const double A = lagrange_A_fast(lg, e);
+        const double B = 1.0 - e;
+        const double D = e * B;
+        const double invP = 1.0 / P;
+        constexpr double sgn = (double)stock::group_sign(m);
+#pragma unroll
+        for (int o = 0; o < 3; ++o) {
+            const double *p = xr[o] + (H - 1);  // p[0] = x[d], p[1] = x[d+1]
+            const double val = A * __builtin_fma(D, S[o] * invP, __builtin_fma(B, p[1], e * p[0]));
+            acc[o] += gbad ? __builtin_nan("") : sgn * val;
+            pin(acc[o]);
+        }
+    });
+    return ok;
+}
+
+// ------------------------------------------------------------------ time-invariant form
+// With equal, constant arms on a uniform time grid the stock TDI is a linear time-invariant
+// filter: the position of chain length m relative to sample i, delta_m = d_m - i + 1 - e_m
+// (the stencil interpolates at sample d_m + 1 - e_m), is the same for every i up to FP64
+// rounding of the times (~1e-9 samples). The groups of one series kind then add up to one
+// composite filter of at most kFir taps, built once per tile from the tile's first sample;
+// each sample checks that its own positions agree within kLtiTol samples (and falls back to
+// tdi_linear otherwise), so the result differs from per-sample weights by at most
+// kLtiTol * |dx/dsample| per stencil -- used by the mixed-precision kernels only.
+constexpr int kFir = 40;
+constexpr double kLtiTol = 1e-8;
+
+template <int NX>
+struct LtiRef {
+    static constexpr int NG = stock::max_len(NX) + 1;
+    int off[NG];       // d_m - i at the reference sample
+    double e[NG];
+    double delta[NG];  // d_m - i + 1 - e_m
+    int kmin;          // first tap of the composite filter, relative to i
+    bool ok;           // the filter fits in kFir taps
+};
+
+template <int NX>
+__device__ __forceinline__ void lti_reference(double t, int i, double L, double t0_offset,
+                                              double fs, LtiRef<NX> &ref)
+{
+    constexpr int H = 13;
+    double Dm = t;
+    int lo = 1 << 30, hi = -(1 << 30);
+    static_for<LtiRef<NX>::NG>([&](auto m) {
+        if constexpr (m > 0) Dm -= L;
+        int d;
+        double e;
+        split_delay(Dm, t0_offset, fs, d, e);
+        ref.off[m] = d - i;
+        ref.e[m] = e;
+        ref.delta[m] = (double)(d - i) + 1.0 - e;
+        lo = min(lo, d - i);
+        hi = max(hi, d - i);
+    });
+    ref.kmin = lo - (H - 1);
+    ref.ok = (hi + H) - ref.kmin + 1 <= kFir;
+}
+
+// Entry idx of the two composite filters: kind idx / kFir, tap kmin + idx % kFir; the groups
+// are added in chain-length order.
+template <int NX>
+__device__ __forceinline__ double lti_filter_entry(const LtiRef<NX> &ref, const LagrangeView &lg,
+                                                   int idx)
+{
+    constexpr int H = 13;
+    const int kind = idx / kFir;
+    const int k = ref.kmin + idx % kFir;
+    double f = 0.0;
+    static_for<LtiRef<NX>::NG>([&](auto m) {
+        if (stock::group_kind(m) == kind) {
+            const int r = k - ref.off[m];
+            if (r >= -(H - 1) && r <= H)
+                f += (double)stock::group_sign(m) * lagrange_weight(lg, ref.e[m], r);
+        }
+    });
+    return f;
+}
+
+// Sample i at time t may use the composite filter: every chain length's position agrees with
+// the reference within kLtiTol and the filter's taps lie in the window.
+template <int NX>
+__device__ __forceinline__ bool lti_matches(double t, int i, double L, double t0_offset,
+                                            double fs, const LtiRef<NX> &ref, int win_lo,
+                                            int win_len)
+{
+    const int first_tap = i + ref.kmin - win_lo;
+    bool ok = first_tap >= 0 && first_tap + kFir <= win_len;
+    double Dm = t;
+    static_for<LtiRef<NX>::NG>([&](auto m) {
+        if constexpr (m > 0) Dm -= L;
+        int d;
+        double e;
+        split_delay(Dm, t0_offset, fs, d, e);
+        const double delta = (double)(d - i) + 1.0 - e;
+        ok = ok && fabs(delta - ref.delta[m]) <= kLtiTol;
+    });
+    return ok;
+}
+
+// acc[o] = sum_k fir[k] s_{o,0}[k] + fir[kFir + k] s_{o,1}[k], s_{o,kind} = row 2*o + kind of
+// the combined-series window starting at column `base`.
+__device__ __forceinline__ void lti_apply(const double *ywin, int cap, int base, const double *fir,
+                                          double (&acc)[3])
+{
+    int off[6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        off[s] = s * cap + base;
+        pin(off[s]);
+    }
+#pragma unroll
+    for (int k = 0; k < kFir; ++k) {
+        const double wa = fir[k];
+        const double wb = fir[kFir + k];
+#pragma unroll
+        for (int o = 0; o < 3; ++o) {
+            acc[o] = __builtin_fma(wa, ywin[off[2 * o] + k], acc[o]);
+            acc[o] = __builtin_fma(wb, ywin[off[2 * o + 1] + k], acc[o]);
+        }
+    }
 }
 
 }  // namespace flr

@@ -17,8 +17,10 @@ INCLUDE_DIR = os.path.join(REPO_ROOT, "include")
 LIB_DIR = os.path.join(PKG_DIR, "lib")
 LIB_PATH = os.path.join(LIB_DIR, "libflr_hip.so")
 
-SOURCES = ["flr_kernels.hip", "flr_capi.hip"]
-HEADERS = ["flr_internal.hpp", "flr_device.hpp"]
+SOURCES = ["flr_kernels.hip", "flr_fused_fp64.hip", "flr_fused_mixed.hip", "flr_capi.hip"]
+HEADERS = ["flr_internal.hpp", "flr_device.hpp", "flr_tdi_stock.hpp", "flr_paths.hpp",
+           "flr_fused.hpp"]
+BUILD_DIR = os.path.join(LIB_DIR, "obj")
 
 # -ffp-contract=off: the time/delay/index arithmetic must round exactly like the reference CPU
 # path; the fast stencils request their fused multiply-adds explicitly.
@@ -43,19 +45,37 @@ def hipcc() -> str:
     return exe
 
 
-def build(verbose: bool = False) -> str:
-    """Compile every HIP source into lib/libflr_hip.so and return its path."""
-    os.makedirs(LIB_DIR, exist_ok=True)
-    srcs = [os.path.join(SRC_DIR, s) for s in SOURCES]
-    tmp = LIB_PATH + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", INCLUDE_DIR, "-o", tmp, *srcs]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
-    proc = subprocess.run(cmd, capture_output=True, text=True)
-    if proc.returncode != 0:
-        raise RuntimeError(f"hipcc failed ({proc.returncode}):\n{proc.stdout}\n{proc.stderr}")
-    if verbose and proc.stderr.strip():
-        print(proc.stderr, file=sys.stderr)
+    return subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+
+
+def build(verbose: bool = False) -> str:
+    """Compile every HIP source (in parallel, one object each) and link lib/libflr_hip.so;
+    returns its path."""
+    os.makedirs(BUILD_DIR, exist_ok=True)
+    compile_flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    objs, procs = [], []
+    for src in SOURCES:
+        obj = os.path.join(BUILD_DIR, os.path.splitext(src)[0] + ".o")
+        objs.append(obj)
+        procs.append((src, _run([hipcc(), *compile_flags, "-I", INCLUDE_DIR, "-c", "-o", obj,
+                                 os.path.join(SRC_DIR, src)], verbose)))
+    errors = []
+    for src, p in procs:
+        out, err = p.communicate()
+        if p.returncode != 0:
+            errors.append(f"{src}: hipcc failed ({p.returncode}):\n{out}\n{err}")
+        elif verbose and err.strip():
+            print(err, file=sys.stderr)
+    if errors:
+        raise RuntimeError("\n".join(errors))
+    tmp = LIB_PATH + ".tmp"
+    p = _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs], verbose)
+    out, err = p.communicate()
+    if p.returncode != 0:
+        raise RuntimeError(f"link failed ({p.returncode}):\n{out}\n{err}")
     os.replace(tmp, LIB_PATH)
     return LIB_PATH
 

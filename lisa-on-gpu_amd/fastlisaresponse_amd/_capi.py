@@ -21,6 +21,7 @@ FLR_FLAG_T_UNIFORM = 0x2
 FLR_FLAG_ZERO_EDGES = 0x4
 FLR_FLAG_OVERWRITE = 0x8
 FLR_FLAG_AET = 0x10
+FLR_FLAG_MIXED = 0x20
 
 # name -> (restype, argtypes); kept in the order of include/flr.h
 _vp, _ci, _cd = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
@@ -44,7 +45,11 @@ SIGNATURES = {
     "flr_tdi": (_ci, [_vp, _vp, _vp, _ci, _vp, _cd, _cd, _cd, _ci, _ci, _vp]),
     "flr_check_errors": (_ci, [_vp, _vp]),
     "flr_batch_create": (_ci, [_vp, _vp, _ci, _vp, _vp, _vp, _cd, _cd, _ci, ctypes.POINTER(_vp)]),
+    "flr_fused_create": (_ci, [_vp, _vp, _ci, _vp, _vp, _cd, _cd, _ci, ctypes.POINTER(_vp)]),
+    "flr_fused_reserve": (_ci, [_vp, _ci, _cd, _cd, _ci, ctypes.POINTER(_vp)]),
+    "flr_fused_bind": (_ci, [_vp, _vp, _ci, _vp, _vp, _vp]),
     "flr_batch_run": (_ci, [_vp, _vp]),
+    "flr_batch_set_launch": (_ci, [_vp, _ci, _ci]),
     "flr_batch_samples": (ctypes.c_int64, [_vp]),
     "flr_batch_destroy": (None, [_vp]),
     "flr_response_set_exact": (_ci, [_vp, _ci]),
@@ -63,6 +68,7 @@ class FlrSource(ctypes.Structure):
         ("k", ctypes.c_double * 3),
         ("u", ctypes.c_double * 3),
         ("v", ctypes.c_double * 3),
+        ("scale", ctypes.c_double),
     ]
 
 _lib = None

@@ -11,6 +11,11 @@ tile results do not depend on their neighbours).
     plan = br.plan(h_all, lengths, lams, betas)     # h_all: packed complex128 device tensor
     plan.run()                                       # enqueues the two launches
     A, E, T = plan.channels(7)                       # views of source 7's output
+
+Fused plans (fused=True, order 25) run projection and TDI as one launch with the projections
+kept in LDS; precision="mixed" additionally stores the input as complex64 h / scale (one scale
+per source, see pack_mixed) and the outputs as float32, with FP64 time/delay arithmetic and
+FP64 stencil arithmetic inside the kernel.
 """
 from __future__ import annotations
 
@@ -27,23 +32,59 @@ from .response import pyResponseTDI
 from .tables import buffer_indices, sky_basis
 
 
+def pack_mixed(h: torch.Tensor, lengths: Sequence[int]):
+    """complex128 packed sources -> (complex64 h / scale, scales): scale = max |h| per source,
+    so the stored samples are O(1) whatever the amplitude."""
+    if not (isinstance(h, torch.Tensor) and h.dtype == torch.complex128):
+        raise TypeError("h must be a complex128 tensor")
+    lengths = [int(n) for n in lengths]
+    if sum(lengths) != h.numel():
+        raise ValueError("lengths do not add up to the number of samples")
+    out = torch.empty(h.numel(), dtype=torch.complex64, device=h.device)
+    scales = []
+    o = 0
+    for n in lengths:
+        seg = h[o:o + n]
+        sc = float(seg.abs().max()) if n else 1.0
+        sc = sc if sc > 0.0 else 1.0
+        out[o:o + n] = (seg / sc).to(torch.complex64)
+        scales.append(sc)
+        o += n
+    return out, scales
+
+
 class BatchedResponse:
     """Response configuration shared by a batch: sampling, order, TDI, orbits, channels.
 
     Args follow pyResponseTDI (sampling_frequency, order, tdi, orbits, tdi_chan, exact) plus
-    t_buffer (s), the edge length left out of TDI for every source."""
+    t_buffer (s), the edge length left out of TDI for every source; fused (one launch, y kept
+    on chip, order 25) and precision ("fp64" or "mixed"; mixed implies fused and the fast
+    stencils). exact defaults to True for "fp64" and False for "mixed"."""
 
     def __init__(self, sampling_frequency: float, order: int = 25, tdi="2nd generation",
-                 orbits=None, tdi_chan: str = "AET", exact: bool = True, t_buffer: float = 10000.0):
+                 orbits=None, tdi_chan: str = "AET", exact: Optional[bool] = None,
+                 t_buffer: float = 10000.0, fused: bool = False, precision: str = "fp64"):
         if tdi_chan not in ("XYZ", "AET", "AE"):
             raise ValueError("tdi_chan must be 'XYZ', 'AET' or 'AE'.")
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        if exact is None:
+            exact = precision == "fp64"
+        if precision == "mixed":
+            if exact:
+                raise ValueError("exact stencils need FP64 storage (precision='fp64')")
+            fused = True
+        if fused and order != 25:
+            raise ValueError("fused batches are built for order 25")
         self.model = pyResponseTDI(sampling_frequency, 1, order=order, tdi=tdi, orbits=orbits,
                                    tdi_chan=tdi_chan, exact=exact)
         self.sampling_frequency = sampling_frequency
         self.dt = self.model.dt
         self.order = order
         self.tdi_chan = tdi_chan
-        self.exact = exact
+        self.exact = bool(exact)
+        self.fused = bool(fused)
+        self.precision = precision
         self.t_buffer = t_buffer
         (self.tdi_start_ind, _, self.projection_buffer,
          self.projections_start_ind) = buffer_indices(sampling_frequency, order, t_buffer,
@@ -52,24 +93,31 @@ class BatchedResponse:
             raise ValueError("Need to increase t_buffer. The initial buffer is not large enough.")
 
     def plan(self, h_all: torch.Tensor, lengths: Sequence[int], lams: Sequence[float],
-             betas: Sequence[float], t0s: Optional[Sequence[float]] = None) -> "BatchPlan":
-        """Bind a packed input (sources back to back in h_all) and allocate the outputs."""
-        return BatchPlan(self, h_all, lengths, lams, betas, t0s)
+             betas: Sequence[float], t0s: Optional[Sequence[float]] = None,
+             scales: Optional[Sequence[float]] = None) -> "BatchPlan":
+        """Bind a packed input (sources back to back in h_all: complex128, or complex64
+        h / scale for precision="mixed") and allocate the outputs."""
+        return BatchPlan(self, h_all, lengths, lams, betas, t0s, scales)
 
 
 class BatchPlan:
     """A bound batch: device source table, packed projections y_all and outputs out_all."""
 
-    def __init__(self, br: BatchedResponse, h_all, lengths, lams, betas, t0s=None):
+    def __init__(self, br: BatchedResponse, h_all, lengths, lams, betas, t0s=None, scales=None):
         lengths = np.asarray(lengths, dtype=np.int64)
         n = len(lengths)
-        if n < 1 or len(lams) != n or len(betas) != n or (t0s is not None and len(t0s) != n):
-            raise ValueError("lengths, lams, betas (and t0s) must have one entry per source")
+        if (n < 1 or len(lams) != n or len(betas) != n or (t0s is not None and len(t0s) != n)
+                or (scales is not None and len(scales) != n)):
+            raise ValueError("lengths, lams, betas (and t0s, scales) must have one entry per source")
         if np.any(lengths < 1):
             raise ValueError("every source needs at least one sample")
-        if not (isinstance(h_all, torch.Tensor) and h_all.is_cuda and h_all.dtype == torch.complex128
+        mixed = br.precision == "mixed"
+        want = torch.complex64 if mixed else torch.complex128
+        if not (isinstance(h_all, torch.Tensor) and h_all.is_cuda and h_all.dtype == want
                 and h_all.is_contiguous()):
-            raise TypeError("h_all must be a contiguous complex128 tensor on the GPU")
+            raise TypeError(f"h_all must be a contiguous {want} tensor on the GPU")
+        if scales is not None and not br.fused:
+            raise ValueError("scales apply to fused plans only")
         total = int(lengths.sum())
         if h_all.numel() != total:
             raise ValueError(f"h_all holds {h_all.numel()} samples, the lengths sum to {total}")
@@ -78,8 +126,9 @@ class BatchPlan:
         self.lengths = lengths
         self.offsets = np.concatenate([[0], np.cumsum(lengths)[:-1]]).astype(np.int64)
         dev = h_all.device
-        self.y_all = torch.empty(6 * total, dtype=torch.float64, device=dev)
-        self.out_all = torch.empty(3 * total, dtype=torch.float64, device=dev)
+        self.y_all = None if br.fused else torch.empty(6 * total, dtype=torch.float64, device=dev)
+        self.out_all = torch.empty(3 * total, dtype=torch.float32 if mixed else torch.float64,
+                                   device=dev)
         srcs = (_capi.FlrSource * n)()
         for s in range(n):
             k, u, v = sky_basis(lams[s], betas[s])
@@ -89,6 +138,7 @@ class BatchPlan:
             q.projections_start_ind = br.projections_start_ind
             q.tdi_start_ind = br.tdi_start_ind
             q.t0 = 0.0 if t0s is None else float(t0s[s])
+            q.scale = 1.0 if scales is None else float(scales[s])
             for c in range(3):
                 q.k[c], q.u[c], q.v[c] = k[c], u[c], v[c]
         flags = _capi.FLR_FLAG_ZERO_EDGES | _capi.FLR_FLAG_OVERWRITE
@@ -96,13 +146,26 @@ class BatchPlan:
             flags |= _capi.FLR_FLAG_AET
         if br.exact:
             flags |= _capi.FLR_FLAG_EXACT
+        if mixed:
+            flags |= _capi.FLR_FLAG_MIXED
         h = ctypes.c_void_p()
-        check(lib().flr_batch_create(br.model.cpp_response._h, srcs, n, h_all.data_ptr(),
-                                     self.y_all.data_ptr(), self.out_all.data_ptr(), br.dt,
-                                     br.sampling_frequency, flags, ctypes.byref(h)))
+        if br.fused:
+            check(lib().flr_fused_create(br.model.cpp_response._h, srcs, n, h_all.data_ptr(),
+                                         self.out_all.data_ptr(), br.dt, br.sampling_frequency,
+                                         flags, ctypes.byref(h)))
+        else:
+            check(lib().flr_batch_create(br.model.cpp_response._h, srcs, n, h_all.data_ptr(),
+                                         self.y_all.data_ptr(), self.out_all.data_ptr(), br.dt,
+                                         br.sampling_frequency, flags, ctypes.byref(h)))
         self._h = h
         self.num_sources = n
         self.samples = int(lib().flr_batch_samples(h))
+        self.launch_shape = (512 if br.fused else 256, 8)  # (tile samples, workgroups per CU)
+
+    def tune(self, **kw) -> dict:
+        """Pick the fastest launch shape for this plan (fastlisaresponse_amd.autotune.tune)."""
+        from .autotune import tune
+        return tune(self, **kw)
 
     def run(self, stream: Optional[int] = None) -> None:
         """Enqueue the batch's projection and TDI launches (no allocation, no sync)."""
@@ -118,6 +181,8 @@ class BatchPlan:
         return (out[0], out[1]) if self.br.tdi_chan == "AE" else (out[0], out[1], out[2])
 
     def projections(self, s: int) -> torch.Tensor:
+        if self.y_all is None:
+            raise ValueError("fused plans keep the projections on chip; use fused=False for y")
         o, n = int(self.offsets[s]), int(self.lengths[s])
         return self.y_all[6 * o: 6 * (o + n)].view(6, n)
 

@@ -16,6 +16,12 @@ returned are torch tensors on the GPU. Differences, all deliberate:
   selects the reciprocal-free fast form, equal to ~1e-15 relative on A and E; the null channel
   T (thousands of times smaller than X, Y, Z for low-frequency sources) then differs at the
   1e-12..1e-11 relative level, since ulp-level differences in X, Y, Z do not cancel in it.
+* `precision="mixed"` (order 25, fast stencils): get_projections stores the input as
+  complex64 h / max|h|; get_tdi_delays runs projection and TDI as one fused launch (the six
+  projections stay on chip) and returns float32 channels. Times, delays and the stencil
+  arithmetic stay FP64 inside the kernel; the error is that of the FP32 storage of h (rel-L2
+  ~1e-7..1e-6 on A, E at mHz frequencies, growing as f^-3 below). `y_gw` is then computed
+  on access, in FP64 from the original input.
 """
 from __future__ import annotations
 
@@ -67,14 +73,25 @@ class pyResponseTDI:
         tdi_orbits (Orbits): stored for interface parity; the kernels use `orbits`.
         tdi_chan (str): "XYZ", "AET" or "AE".
         force_backend: None or a GPU backend name.
-        exact (bool): reference stencil operation order, bitwise FP64 CPU parity (default);
-            False selects the faster reciprocal-free stencils.
+        exact (bool): reference stencil operation order, bitwise FP64 CPU parity (default for
+            precision "fp64"); False selects the faster reciprocal-free stencils.
+        precision (str): "fp64" (default) or "mixed" (FP32 storage, fused launch, order 25).
     """
 
     def __init__(self, sampling_frequency, num_pts, order=25, tdi="1st generation",
                  orbits: Optional[Orbits] = None, tdi_orbits: Optional[Orbits] = None,
-                 tdi_chan="XYZ", force_backend=None, exact: bool = True):
+                 tdi_chan="XYZ", force_backend=None, exact: Optional[bool] = None,
+                 precision: str = "fp64"):
         _check_backend(force_backend)
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        if exact is None:
+            exact = precision == "fp64"
+        if precision == "mixed" and exact:
+            raise ValueError("exact stencils need precision='fp64'")
+        if precision == "mixed" and order != 25:
+            raise ValueError("precision='mixed' is built for order 25")
+        self.precision = precision
         self.sampling_frequency = sampling_frequency
         self.dt = 1 / sampling_frequency
         self.tdi_buffer = 200
@@ -114,6 +131,7 @@ class pyResponseTDI:
         self.y_gw_length = None
         self.delayed_links_flat = None
         self._xyz_stale = False
+        self._mixed = None  # precision="mixed": (h / scale as complex64, scale, k, u, v)
 
     # ------------------------------------------------------------ properties
     @property
@@ -212,7 +230,19 @@ class pyResponseTDI:
     @property
     def y_gw(self) -> torch.Tensor:
         """Projections along the arms, [6, num_pts], rows in orbits.LINKS order."""
+        if self.y_gw_flat is None and self._mixed is not None:
+            self._project(self._input_keepalive, self._mixed[2:], self.t0_projection)
         return self.y_gw_flat.reshape(self.nlinks, -1)
+
+    def _project(self, h: torch.Tensor, basis, t0: float) -> None:
+        k, u, v = basis
+        y_gw = torch.empty(self.nlinks * self.num_pts, dtype=torch.float64, device=self.device)
+        flags = _capi.FLR_FLAG_ZERO_EDGES | (_capi.FLR_FLAG_EXACT if self.exact else 0)
+        self.cpp_response.projection(y_gw, h, self.num_pts, float(t0), self.dt,
+                                     self.sampling_frequency, k, u, v,
+                                     self.projections_start_ind, flags)
+        self.y_gw_flat = y_gw
+        self.y_gw_length = self.num_pts
 
     def _data_time_check(self, t0: float, n: int, input_in):
         t_last = t0 + (n - 1) * self.dt
@@ -258,14 +288,15 @@ class pyResponseTDI:
             raise ValueError(f"y_gw: input arrays have the incorrect length. Should be "
                              f"{6 * len(h)}. It's length is {6 * self.num_pts}.")
         h = h.to(device=self.device, dtype=torch.complex128).contiguous()
-        y_gw = torch.empty(self.nlinks * self.num_pts, dtype=torch.float64, device=self.device)
-        flags = _capi.FLR_FLAG_ZERO_EDGES | (_capi.FLR_FLAG_EXACT if self.exact else 0)
-        self.cpp_response.projection(y_gw, h, self.num_pts, float(t0), self.dt,
-                                     self.sampling_frequency, k, u, v,
-                                     self.projections_start_ind, flags)
-        self.y_gw_flat = y_gw
-        self.y_gw_length = self.num_pts
         self._input_keepalive = h
+        if self.precision == "mixed":
+            scale = float(h.abs().max())
+            scale = scale if scale > 0.0 else 1.0
+            self._mixed = ((h / scale).to(torch.complex64).contiguous(), scale, k, u, v)
+            self.y_gw_flat = None
+            self.y_gw_length = self.num_pts
+            return
+        self._project(h, (k, u, v), t0)
 
     # ------------------------------------------------------------ TDI
     @property
@@ -275,7 +306,43 @@ class pyResponseTDI:
             self._run_tdi(self._tdi_t0, aet=False)
         return self.delayed_links_flat.reshape(3, -1)
 
+    def _run_fused(self, t0: float, aet: bool) -> torch.Tensor:
+        """precision="mixed": projection + TDI in one launch from the stored complex64 input."""
+        import ctypes
+        h32, scale, k, u, v = self._mixed
+        out = torch.empty(3 * self.num_pts, dtype=torch.float32, device=self.device)
+        srcs = (_capi.FlrSource * 1)()
+        q = srcs[0]
+        q.offset = 0
+        q.num_pts = self.num_pts
+        q.projections_start_ind = self.projections_start_ind
+        q.tdi_start_ind = self.tdi_start_ind
+        q.t0 = float(t0)
+        q.scale = scale
+        for c in range(3):
+            q.k[c], q.u[c], q.v[c] = k[c], u[c], v[c]
+        flags = _capi.FLR_FLAG_ZERO_EDGES | _capi.FLR_FLAG_OVERWRITE | _capi.FLR_FLAG_MIXED
+        if aet:
+            flags |= _capi.FLR_FLAG_AET
+        L = _capi.lib()
+        hb = ctypes.c_void_p()
+        _capi.check(L.flr_fused_create(self.cpp_response._h, srcs, 1, h32.data_ptr(),
+                                       out.data_ptr(), self.dt, self.sampling_frequency, flags,
+                                       ctypes.byref(hb)))
+        try:
+            from .backend import current_stream
+            _capi.check(L.flr_batch_run(hb, current_stream()))
+            self.cpp_response.check_errors()  # synchronises before the source table is freed
+        finally:
+            L.flr_batch_destroy(hb)
+        if not aet:
+            self.delayed_links_flat = out
+            self._xyz_stale = False
+        return out
+
     def _run_tdi(self, t0: float, aet: bool) -> torch.Tensor:
+        if self._mixed is not None:
+            return self._run_fused(t0, aet)
         out = torch.empty(3 * self.y_gw_length, dtype=torch.float64, device=self.device)
         flags = _capi.FLR_FLAG_ZERO_EDGES | _capi.FLR_FLAG_OVERWRITE
         if aet:
@@ -308,10 +375,11 @@ class pyResponseTDI:
                 raise ValueError(f"y_gw must have shape (6, {self.num_pts})")
             self.y_gw_flat = y.to(device=self.device, dtype=torch.float64).reshape(-1).clone()
             self.y_gw_length = self.num_pts
+            self._mixed = None  # given projections: FP64 TDI from them
             if not hasattr(self, "tdi_start_ind"):
                 raise ValueError("get_projections sets tdi_start_ind; run it first or set "
                                  "tdi_start_ind = int(t_buffer / dt)")
-        elif self.y_gw_flat is None:
+        elif self.y_gw_flat is None and self._mixed is None:
             raise ValueError("Need to either enter projection array or have this code determine "
                              "projections.")
         if t0 is None:

@@ -1,0 +1,235 @@
+"""Fused projection + TDI launches (y kept in LDS) and the mixed-precision form, on the GPU.
+
+* Fused exact FP64 is bitwise the unfused exact path (and so the FP64 oracle).
+* Fused fast FP64 (one stencil per output and chain length for equal arms) is within the fast
+  form's tolerance of the oracle.
+* Mixed precision stores h as complex64 h / scale and the outputs as float32; inside, times,
+  delays and stencils are FP64. It is pinned two ways: against the oracle run on the same
+  complex64-rounded input (isolates the kernel: only the fast form's ~1e-15 and the float32
+  output rounding remain, rel-L2 <= 2e-7 per channel), and against the oracle on the FP64
+  input (the storage error: rel-L2 <= 1e-6 on A and E at 5 mHz).
+"""
+import numpy as np
+import pytest
+
+from conftest import YRSID_SI, gb_waveform, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL_FAST = 1e-12        # fast form vs oracle, A and E (T: absolute, relative to the A/E scale)
+TOL_MIXED_KERNEL = 2e-7  # mixed kernel vs oracle on the same complex64-rounded input
+TOL_MIXED_5MHZ = 1e-6    # mixed vs oracle on the FP64 input, A and E, f0 = 5 mHz
+
+
+@pytest.fixture(scope="module")
+def torch_gpu():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+@pytest.fixture(scope="module", params=["equal", "unequal"])
+def orbits(request):
+    from fastlisaresponse_amd.orbits import CartwheelOrbits, EqualArmlengthOrbits
+    cls = EqualArmlengthOrbits if request.param == "equal" else CartwheelOrbits
+    o = cls(t_max=1.2 * YRSID_SI)
+    o.configure(linear_interp_setup=True)
+    return o
+
+
+@pytest.fixture(scope="module")
+def oracle_orbits(orbits, oracle_mod):
+    return oracle_mod.OracleOrbits(*orbits.pycppdetector_args)
+
+
+def _sources(rng, lengths):
+    params, hs = [], []
+    for n in lengths:
+        f = 10 ** rng.uniform(-3, -2)
+        fd = 10 ** rng.uniform(-19, -16)
+        A = 10 ** rng.uniform(-23, -21)
+        iota = np.arccos(rng.uniform(-1, 1))
+        phi0, psi = rng.uniform(0, 2 * np.pi, 2)
+        lam = rng.uniform(0, 2 * np.pi)
+        beta = np.arcsin(rng.uniform(-1, 1))
+        t0 = float(rng.choice([0.0, 1.0e6, 2.5e7]))
+        hs.append(gb_waveform(n, 10.0, A, f, fd, iota, phi0, psi, t0=t0))
+        params.append((lam, beta, t0))
+    return hs, params
+
+
+def _plan(torch, br, hs, params, scales=None, h_all=None):
+    if h_all is None:
+        h_all = torch.as_tensor(np.concatenate(hs), device="cuda")
+    return br.plan(h_all, [len(h) for h in hs], [p[0] for p in params], [p[1] for p in params],
+                   [p[2] for p in params], scales=scales)
+
+
+def _fast_ok(got, exp, aet):
+    if not aet:
+        for c in range(3):
+            assert rel_l2(got[c], exp[c]) <= TOL_FAST, (c, rel_l2(got[c], exp[c]))
+        return
+    scale = max(np.linalg.norm(exp[0]), np.linalg.norm(exp[1]))
+    for c in range(2):
+        assert rel_l2(got[c], exp[c]) <= TOL_FAST, (c, rel_l2(got[c], exp[c]))
+    assert np.linalg.norm(got[2] - exp[2]) <= TOL_FAST * scale
+
+
+@pytest.mark.parametrize("tdi,chan", [("2nd generation", "AET"), ("1st generation", "XYZ")])
+def test_fused_exact_is_bitwise_the_unfused_path(torch_gpu, orbits, oracle_orbits, oracle_mod,
+                                                 tdi, chan):
+    torch = torch_gpu
+    from fastlisaresponse_amd.batch import BatchedResponse
+    rng = np.random.default_rng(31)
+    lengths = [5000, 12345, 2100, 3000, 20000, 7777]  # 2100: only edges
+    hs, params = _sources(rng, lengths)
+    ref = _plan(torch, BatchedResponse(0.1, tdi=tdi, orbits=orbits, tdi_chan=chan), hs, params)
+    fused = _plan(torch, BatchedResponse(0.1, tdi=tdi, orbits=orbits, tdi_chan=chan, fused=True),
+                  hs, params)
+    ref.run()
+    fused.run()
+    ref.check_errors()
+    fused.check_errors()
+    assert torch.equal(fused.out_all, ref.out_all)
+    lam, beta, t0 = params[4]
+    _, exp = oracle_mod.response_pipeline(oracle_orbits, hs[4], lam, beta, 10.0, 25, tdi, t0=t0,
+                                          tdi_chan=chan, x_base=orbits.x_base, nthreads=8)
+    np.testing.assert_array_equal(torch.stack(fused.channels(4)).cpu().numpy(), exp)
+
+
+@pytest.mark.parametrize("tdi,chan", [("2nd generation", "AET"), ("1st generation", "XYZ")])
+def test_fused_fast_fp64_vs_oracle(torch_gpu, orbits, oracle_orbits, oracle_mod, tdi, chan):
+    torch = torch_gpu
+    from fastlisaresponse_amd.batch import BatchedResponse
+    rng = np.random.default_rng(37)
+    lengths = [40000, 9000]
+    hs, params = _sources(rng, lengths)
+    br = BatchedResponse(0.1, tdi=tdi, orbits=orbits, tdi_chan=chan, fused=True, exact=False)
+    plan = _plan(torch, br, hs, params)
+    plan.run()
+    plan.check_errors()
+    for s in range(len(lengths)):
+        lam, beta, t0 = params[s]
+        _, exp = oracle_mod.response_pipeline(oracle_orbits, hs[s], lam, beta, 10.0, 25, tdi,
+                                              t0=t0, tdi_chan=chan, x_base=orbits.x_base,
+                                              nthreads=8)
+        got = torch.stack(plan.channels(s)).cpu().numpy()
+        T = br.tdi_start_ind
+        assert np.all(got[:, :T] == 0.0) and np.all(got[:, -T:] == 0.0)
+        _fast_ok(got[:, T:-T], exp[:, T:-T], chan == "AET")
+
+
+def test_mixed_kernel_vs_oracle_on_rounded_input(torch_gpu, orbits, oracle_orbits, oracle_mod):
+    """The kernel itself: oracle on the same complex64 samples (times scale) in FP64."""
+    torch = torch_gpu
+    from fastlisaresponse_amd.batch import BatchedResponse, pack_mixed
+    rng = np.random.default_rng(41)
+    lengths = [60000, 11000, 25000]
+    hs, params = _sources(rng, lengths)
+    h64 = torch.as_tensor(np.concatenate(hs), device="cuda")
+    h32, scales = pack_mixed(h64, lengths)
+    br = BatchedResponse(0.1, tdi="2nd generation", orbits=orbits, tdi_chan="AET",
+                         precision="mixed")
+    plan = _plan(torch, br, hs, params, scales=scales, h_all=h32)
+    plan.run()
+    plan.check_errors()
+    assert plan.out_all.dtype == torch.float32
+    h32_host = h32.cpu().numpy()
+    o = 0
+    for s, n in enumerate(lengths):
+        lam, beta, t0 = params[s]
+        hq = h32_host[o:o + n].astype(np.complex128) * scales[s]
+        o += n
+        _, exp = oracle_mod.response_pipeline(oracle_orbits, hq, lam, beta, 10.0, 25,
+                                              "2nd generation", t0=t0, x_base=orbits.x_base,
+                                              nthreads=8)
+        got = torch.stack(plan.channels(s)).cpu().numpy().astype(np.float64)
+        for c in range(3):
+            r = rel_l2(got[c, 1000:-1000], exp[c, 1000:-1000])
+            assert r <= TOL_MIXED_KERNEL, (s, c, r)
+
+
+def test_mixed_vs_fp64_oracle_at_5mhz(torch_gpu, orbits, oracle_orbits, oracle_mod):
+    """End to end through pyResponseTDI(precision="mixed"): the FP32 storage error."""
+    torch = torch_gpu
+    from fastlisaresponse_amd import pyResponseTDI
+    n = 1 << 18
+    h = gb_waveform(n, 10.0, 1e-21, 5e-3, 1e-16, 0.8, 0.3, 0.5)
+    r = pyResponseTDI(0.1, n, orbits=orbits, tdi="2nd generation", tdi_chan="AET",
+                      precision="mixed")
+    r.get_projections(h, 1.2, 0.3)
+    A, E, T = r.get_tdi_delays()
+    assert A.dtype == torch.float32
+    got = torch.stack([A, E, T]).cpu().numpy().astype(np.float64)
+    _, exp = oracle_mod.response_pipeline(oracle_orbits, h, 1.2, 0.3, 10.0, 25, "2nd generation",
+                                          x_base=orbits.x_base, nthreads=8)
+    for c in range(2):
+        r_c = rel_l2(got[c, 1000:-1000], exp[c, 1000:-1000])
+        assert r_c <= TOL_MIXED_5MHZ, (c, r_c)
+    # the null channel T is ~1e3 below A, E here: its relative error is ~1e-4
+    assert rel_l2(got[2, 1000:-1000], exp[2, 1000:-1000]) <= 1e-3
+    # the XYZ view and y_gw are available (FP64, from the original input)
+    xyz = r.XYZ.cpu().numpy().astype(np.float64)
+    y_exp, xyz_exp = oracle_mod.response_pipeline(oracle_orbits, h, 1.2, 0.3, 10.0, 25,
+                                                  "2nd generation", tdi_chan="XYZ",
+                                                  x_base=orbits.x_base, nthreads=8)
+    for c in range(3):
+        assert rel_l2(xyz[c, 1000:-1000], xyz_exp[c, 1000:-1000]) <= TOL_MIXED_5MHZ
+    y = r.y_gw.cpu().numpy()
+    for l in range(6):
+        assert rel_l2(y[l], y_exp[l]) <= TOL_FAST
+
+
+def test_fused_argument_checks(torch_gpu, orbits):
+    torch = torch_gpu
+    from fastlisaresponse_amd import pyResponseTDI
+    from fastlisaresponse_amd.batch import BatchedResponse
+    with pytest.raises(ValueError):
+        BatchedResponse(0.1, orbits=orbits, precision="mixed", exact=True)
+    with pytest.raises(ValueError):
+        BatchedResponse(0.1, orbits=orbits, order=15, fused=True)
+    with pytest.raises(ValueError):
+        pyResponseTDI(0.1, 1000, orbits=orbits, precision="mixed", order=15)
+    br = BatchedResponse(0.1, orbits=orbits, precision="mixed")
+    h = torch.zeros(5000, dtype=torch.complex128, device="cuda")
+    with pytest.raises(TypeError):
+        br.plan(h, [5000], [0.1], [0.2])  # mixed plans take complex64
+    plan = br.plan(h.to(torch.complex64), [5000], [0.1], [0.2])
+    with pytest.raises(ValueError):
+        plan.projections(0)
+
+
+@pytest.mark.parametrize("kind", ["mixed", "fused-fp64", "unfused"])
+def test_launch_shapes_give_identical_results(torch_gpu, orbits, kind):
+    """Tile length and grid density change speed only: every shape the autotuner may pick gives
+    the same bits."""
+    torch = torch_gpu
+    from fastlisaresponse_amd.autotune import apply, candidates
+    from fastlisaresponse_amd.batch import BatchedResponse, pack_mixed
+    rng = np.random.default_rng(43)
+    lengths = [30000, 4000, 12000]
+    hs, params = _sources(rng, lengths)
+    kw = dict(mixed=dict(precision="mixed"), unfused=dict(),
+              **{"fused-fp64": dict(fused=True, exact=False)})[kind]
+    br = BatchedResponse(0.1, tdi="2nd generation", orbits=orbits, tdi_chan="AET", **kw)
+    scales = None
+    h_all = None
+    if kind == "mixed":
+        h_all, scales = pack_mixed(torch.as_tensor(np.concatenate(hs), device="cuda"), lengths)
+    plan = _plan(torch, br, hs, params, scales=scales, h_all=h_all)
+    ref = None
+    shapes = list(candidates(plan))
+    assert len(shapes) == (9 if kind == "mixed" else 3)
+    for shape in shapes:
+        apply(plan, *shape)
+        plan.out_all.fill_(float("nan"))
+        plan.run()
+        plan.check_errors()
+        if ref is None:
+            ref = plan.out_all.clone()
+        else:
+            assert torch.equal(plan.out_all, ref), shape
+    res = plan.tune(use_cache=False, reps=1)
+    assert tuple(res["best"]) in shapes and plan.launch_shape == tuple(res["best"])

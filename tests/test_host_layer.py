@@ -101,3 +101,27 @@ def test_overlap_helper():
     a = [rng.normal(size=256) for _ in range(3)]
     assert abs(get_overlap(a, a) - 1.0) < 1e-12
     assert abs(get_overlap(a, [-x for x in a]) + 1.0) < 1e-12
+
+
+def test_poisson_traffic_mix():
+    """Serving traffic generator: Poisson arrivals, 1..64 sources, lengths, dt and the 30/70
+    TDI mix of the serving workload (host-side only: the pool is a list of CPU tensors)."""
+    import numpy as np
+    import torch
+    from fastlisaresponse_amd.serving import poisson_requests
+    pool = [torch.zeros(1 << 12, dtype=torch.complex64) for _ in range(2)]
+    rng = np.random.default_rng(5)
+    times, reqs = poisson_requests(rng, 200.0, 20.0, pool, lengths=(1 << 10, 1 << 11, 1 << 12))
+    n = len(reqs)
+    assert abs(n - 4000) < 4 * np.sqrt(4000)
+    assert np.all(np.diff(times) > 0) and times[-1] < 20.0
+    gaps = np.diff([0.0] + list(times))
+    assert abs(gaps.mean() - 1 / 200.0) < 0.1 / 200.0
+    second = sum(r.tdi == "2nd generation" for r in reqs) / n
+    assert abs(second - 0.7) < 0.03
+    assert {r.dt for r in reqs} == {5.0, 10.0}
+    ns = [len(r.sources) for r in reqs]
+    assert min(ns) == 1 and max(ns) == 64
+    for r in reqs[:50]:
+        assert len({s.num_pts for s in r.sources}) == 1
+        assert all(-23 <= np.log10(s.scale) <= -21 for s in r.sources)
