@@ -15,6 +15,8 @@ Also on the same JSON line:
                 multiply-adds, 3744 FLOP per 2nd-generation sample) against the 78.6 TFLOP/s
                 FP64 vector peak, its launch time from HIP events in the timed region, and HBM
                 bytes per launch from the committed rocprofv3 counter summary
+  batch         the 512-source batch (durations U[0.05, 1] yr): FP64 exact vs autotuned
+                mixed-precision samples/s, and the mixed form's per-source rel-L2
   latency       p50/p99 of a short open-loop serving run: Poisson arrivals at 200 req/s of
                 1..64-source requests (2^18/2^20/2^22 samples, dt 5/10 s, 30/70 1st/2nd gen)
   parity        rel-L2 / max-abs of A, E, T against the FP64 oracle on the full workload
@@ -67,16 +69,19 @@ def gb_waveform(n, dt, A=1e-21, f=5e-3, fdot=1e-16, iota=0.8, phi0=0.3, psi=0.5)
     return hp + 1j * hc
 
 
-def pmc_traffic(name: str):
+def pmc_traffic(name: str, variant: str = ""):
     """HBM bytes per launch from a committed rocprofv3 --pmc summary (FETCH_SIZE and
-    WRITE_SIZE, calibrated against a copy kernel), or None if it does not exist."""
+    WRITE_SIZE, calibrated against a copy kernel): the HBM_BYTES_PER_LAUNCH line (of
+    `variant`, when the summary has one per variant), or None if there is none."""
     path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as f:
-            m = re.search(r"^HBM_BYTES_PER_LAUNCH\s+([0-9.eE+]+)", f.read(), re.M)
-        return float(m.group(1)) if m else None
+            text = f.read()
     except OSError:
         return None
+    key = r"HBM_BYTES_PER_LAUNCH" + (r"\[" + re.escape(variant) + r"\]" if variant else "")
+    m = re.search(r"^" + key + r"\s+([0-9.eE+]+)", text, re.M)
+    return float(m.group(1)) if m else None
 
 
 def cpu_baseline(orbits, h, lam, beta, tdi, n_sample):
@@ -110,6 +115,91 @@ def timed(fn, steps, warmup):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     return wall, sum(a.elapsed_time(b) for a, b in ev) / steps
+
+
+def batch_run(num_sources, steps, seed=3):
+    """The 512-source batch: durations U[0.05, 1] yr at dt = 10 s, sky uniform on the sphere,
+    psi, phi0 U[0, 2 pi), cos i U[-1, 1], f0 log-U[1e-4, 1e-2] Hz, fdot log-U[1e-19, 1e-15] Hz/s,
+    log10 A U[-23, -21] (numpy default_rng(3)); 2nd-gen AET, order 25. Times the FP64 exact
+    batch (two persistent launches) and the autotuned mixed-precision fused batch; reports the
+    mixed form's rel-L2 per source against the FP64 exact result (bitwise the oracle's)."""
+    import numpy as np
+    import torch
+    from fastlisaresponse_amd.batch import BatchedResponse, pack_mixed
+    from fastlisaresponse_amd.orbits import EqualArmlengthOrbits
+    rng = np.random.default_rng(seed)
+    orb = EqualArmlengthOrbits(t_max=1.2 * YRSID_SI)
+    orb.configure(linear_interp_setup=True)
+    lengths = (rng.uniform(0.05, 1.0, num_sources) * YRSID_SI / DT).astype(np.int64)
+    lam = rng.uniform(0, 2 * np.pi, num_sources)
+    beta = np.arcsin(rng.uniform(-1, 1, num_sources))
+    psi = rng.uniform(0, 2 * np.pi, num_sources)
+    phi0 = rng.uniform(0, 2 * np.pi, num_sources)
+    ci = rng.uniform(-1, 1, num_sources)
+    f0 = 10 ** rng.uniform(-4, -2, num_sources)
+    fd = 10 ** rng.uniform(-19, -15, num_sources)
+    amp = 10 ** rng.uniform(-23, -21, num_sources)
+    total = int(lengths.sum())
+    h = torch.empty(total, dtype=torch.complex128, device="cuda")
+    o = 0
+    for s in range(num_sources):  # GB generator of the reference's test, on the device
+        n = int(lengths[s])
+        t = torch.arange(n, dtype=torch.float64, device="cuda") * DT
+        fdd = 11.0 / 3.0 * fd[s] ** 2 / f0[s]
+        ph = 2 * np.pi * (f0[s] * t + 0.5 * fd[s] * t * t + fdd * t ** 3 / 6.0) - phi0[s]
+        hp0 = -torch.cos(ph) * amp[s] * (1.0 + ci[s] ** 2)
+        hc0 = -torch.sin(ph) * 2.0 * amp[s] * ci[s]
+        c2, s2 = np.cos(2 * psi[s]), np.sin(2 * psi[s])
+        h[o:o + n] = torch.complex(hp0 * c2 - hc0 * s2, hp0 * s2 + hc0 * c2)
+        o += n
+    torch.cuda.synchronize()
+    res = {"sources": num_sources, "samples": total, "seed": seed,
+           "workload": "512 galactic binaries, durations U[0.05,1] yr at dt=10 s, 2nd-gen AET"}
+
+    def time_plan(plan):
+        return timed(plan.run, steps, 1)
+
+    ref = BatchedResponse(1 / DT, tdi="2nd generation", orbits=orb, precision="fp64")
+    p64 = ref.plan(h, lengths, lam, beta)
+    res["fp64_tune"] = p64.tune(use_cache=False, reps=1)["best"]
+    wall, ms = time_plan(p64)
+    p64.check_errors()
+    res["fp64_exact_samples_per_s"] = total * steps / wall
+    out64 = p64.out_all.view(-1)
+    del p64
+
+    mb = BatchedResponse(1 / DT, tdi="2nd generation", orbits=orb, precision="mixed")
+    h32, scales = pack_mixed(h, lengths)
+    del h
+    pm = mb.plan(h32, lengths, lam, beta, scales=scales)
+    tune = pm.tune(use_cache=False, reps=2)
+    res["mixed_tune"] = {"best": tune["best"], "ms": tune["ms"]}
+    wall, ms = time_plan(pm)
+    pm.check_errors()
+    res["mixed_samples_per_s"] = total * steps / wall
+    res["mixed_vs_fp64"] = res["mixed_samples_per_s"] / res["fp64_exact_samples_per_s"]
+    # per-source rel-L2 of the mixed form against the FP64 exact batch
+    T = int(T_BUFFER / DT)
+    rel = np.zeros((num_sources, 3))
+    o = 0
+    outm = pm.out_all.view(-1)
+    for s in range(num_sources):
+        n = int(lengths[s])
+        a = out64[3 * o: 3 * (o + n)].view(3, n)[:, T:n - T]
+        b = outm[3 * o: 3 * (o + n)].view(3, n)[:, T:n - T].double()
+        rel[s] = ((b - a).norm(dim=1) / a.norm(dim=1)).cpu().numpy()
+        o += n
+    res["mixed_rel_l2_vs_fp64"] = {
+        ch: {"median": float(np.median(rel[:, c])), "p90": float(np.percentile(rel[:, c], 90)),
+             "max": float(rel[:, c].max()),
+             "sources_within_1e-6": int((rel[:, c] <= 1e-6).sum())}
+        for c, ch in enumerate("AET")}
+    by_f = {}
+    for lo in (-4, -3):
+        sel = (np.log10(f0) >= lo) & (np.log10(f0) < lo + 1)
+        by_f[f"f0 in [1e{lo}, 1e{lo + 1})"] = float(np.median(rel[sel, 0])) if sel.any() else None
+    res["mixed_rel_l2_A_median_by_f0"] = by_f
+    return res
 
 
 def serving_run(seconds, rate, seed=5):
@@ -153,6 +243,11 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=2.0,
                     help="seconds of Poisson arrivals in the serving run (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="requests per second")
+    ap.add_argument("--batch-sources", type=int, default=512,
+                    help="sources of the batched workload (0 skips it)")
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--launch", default=None, metavar="TILExGRID",
+                    help="fixed launch shape of the mixed plan (e.g. 512x32) instead of autotuning")
     ap.add_argument("--only", choices=["mixed", "fp64"], default=None,
                     help="time one form only (for profiling)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -196,13 +291,24 @@ def main() -> int:
                              precision="mixed")
         h32, scales = pack_mixed(h, [N_SAMPLES])
         plan = br.plan(h32, [N_SAMPLES], [lam], [beta], scales=scales)
+        if args.launch:
+            from fastlisaresponse_amd.autotune import apply
+            tile, grid = (int(x) for x in args.launch.lower().split("x"))
+            apply(plan, tile, grid)
+            line["config"]["launch_shape"] = {"tile_window": tile, "workgroups_per_cu": grid}
+        else:
+            tune = plan.tune(use_cache=False, reps=3)
+            line["config"]["launch_shape"] = {"tile_window": tune["best"][0],
+                                              "workgroups_per_cu": tune["best"][1],
+                                              "autotune_ms": tune["ms"]}
         wall, ms = timed(plan.run, args.steps, args.warmup)
         plan.check_errors()
         out_mixed = plan.out_all.view(3, -1)
         line.update({"value": N_SAMPLES * args.steps / wall, "ms_per_step": wall * 1e3 / args.steps})
         achieved = flop * N_SAMPLES / (ms * 1e-3) / 1e12
+        tile = line["config"]["launch_shape"]["tile_window"]
         traffic = pmc_traffic(f"fused_mixed_1yr_{'2nd' if args.tdi.startswith('2nd') else '1st'}"
-                              f"_gen_{args.orbits}_pmc.txt")
+                              f"_gen_{args.orbits}_pmc.txt", f"tile{tile}")
         line["roofline"] = {
             "bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
@@ -251,6 +357,9 @@ def main() -> int:
                 "max_abs_AET": [float(np.abs(got[c] - aet_ref[c]).max()) for c in range(3)],
                 "bitwise_equal": bool(np.array_equal(got, aet_ref))}
         line["parity"] = par
+
+    if args.batch_sources > 0 and args.only is None:
+        line["batch"] = batch_run(args.batch_sources, args.batch_steps)
 
     if args.serve_seconds > 0 and args.only is None:
         s = serving_run(args.serve_seconds, args.serve_rate)
