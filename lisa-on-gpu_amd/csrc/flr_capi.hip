@@ -151,7 +151,7 @@ struct flr_batch {
     int capacity = 0;             // fused: sources the tables hold
     int tile_passes = kFusedPasses;  // fused: the tile's window is tile_passes * kBlock samples
     int tile_len = 0;                // fused: output samples per tile (make_fused_config)
-    int grid_per_cu = 8;             // persistent workgroups per CU
+    int grid_per_cu = 16;            // persistent workgroups per CU
     FusedSource *h_fused = nullptr;  // pinned staging of the source table and tile prefix
     int *h_fused_tiles = nullptr;
     hipEvent_t uploaded = nullptr;   // the last bind's upload
@@ -333,16 +333,13 @@ int make_proj_config(const flr_response *r, double dt, double fs, int flags, Pro
     cfg.dt = dt;
     cfg.fs = fs;
     cfg.zero_edges = (flags & FLR_FLAG_ZERO_EDGES) ? 1 : 0;
-    for (int s = 0; s < 3; ++s) {
-        int q = 0;
-        for (int row = 0; row < kNLinks; ++row) {
-            if (o->sc_r[row] != s + 1) continue;
-            if (q == 2) return fail(FLR_ERR_INVALID_ARGUMENT, "a spacecraft receives more than two links");
-            cfg.rec_row[s][q] = row;
-            cfg.em_sc[s][q] = o->sc_e[row] - 1;
-            ++q;
-        }
-        if (q != 2) return fail(FLR_ERR_INVALID_ARGUMENT, "every spacecraft must receive two links");
+    for (int s = 0; s < 3; ++s)
+        for (int e = 0; e < 3; ++e) cfg.link_row[s][e] = -1;
+    for (int row = 0; row < kNLinks; ++row) {
+        const int s = o->sc_r[row] - 1, e = o->sc_e[row] - 1;
+        if (s < 0 || s > 2 || e < 0 || e > 2 || s == e || cfg.link_row[s][e] >= 0)
+            return fail(FLR_ERR_INVALID_ARGUMENT, "links must be the six ordered spacecraft pairs");
+        cfg.link_row[s][e] = row;
     }
     const double rc = o->max_radius * kCInv;
     cfg.halo_lo = (int)std::ceil((o->max_ltt + rc) * fs) + h + 4;
@@ -438,7 +435,7 @@ int projection_impl(flr_response *r, double *y_gw, const double *input_in, int n
         src.v[c] = v[c];
     }
     hipError_t e = launch_projection(cfg, src, r->orbits->view(), r->lagrange(),
-                                     (flags & FLR_FLAG_EXACT) != 0, stream);
+                                     (flags & FLR_FLAG_EXACT) != 0, r->orbits->equal_arm, stream);
     if (e != hipSuccess) return hip_fail(e, "projection launch");
     return FLR_OK;
 }
@@ -1000,7 +997,7 @@ int flr_batch_run(flr_batch *b, void *stream)
     Batch<ProjSource> bp{b->d_proj, b->d_proj_tiles, b->num_sources, b->proj_tiles};
     Batch<TdiSource> bt{b->d_tdi, b->d_tdi_tiles, b->num_sources, b->tdi_tiles};
     hipError_t e = launch_projection_batch(pc, bp, r->orbits->view(), r->lagrange(), exact,
-                                           b->grid_per_cu, s);
+                                           r->orbits->equal_arm, b->grid_per_cu, s);
     if (e != hipSuccess) return hip_fail(e, "batch projection launch");
     e = launch_tdi_batch(tc, bt, r->orbits->view(), r->lagrange(), exact, stock, b->grid_per_cu, s);
     if (e != hipSuccess) return hip_fail(e, "batch tdi launch");

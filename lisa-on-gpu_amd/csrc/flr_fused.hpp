@@ -110,7 +110,7 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
             for (int r = 0; r < kNLinks; ++r) yc[r * cap] = 0.0;
         } else {
             const double t = src.t0 + (double)j * pc.dt;
-            bad |= project_links<EXACT, H>(pc, src.k, src.u, src.v, t, t0_offset, orb, lg, hwin,
+            bad |= project_links<EXACT, H, MODE != kModeGeneric>(pc, src.k, src.u, src.v, t, t0_offset, orb, lg, hwin,
                                            hw_lo, hw_len, y_deps,
                                            [&](int row, double val) { yc[row * cap] = val; });
         }
@@ -118,15 +118,17 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
     }
     __syncthreads();
 
-    // Mixed precision, equal arms: composite time-invariant filters for the tile.
+    // Mixed precision, equal arms: composite time-invariant filters, referenced at the
+    // source's first TDI sample so that every tile of a source (whatever the tile length or
+    // the batch) applies the same filter.
     constexpr bool kLti = MODE == kModeLinear && std::is_same<OT, float>::value;
     double *fir = ywin + kNLinks * cap;  // [2][kFir]
     LtiRef<NX> lref;
     bool lti = false;
     if constexpr (kLti) {
-        const double t_ref = src.t0 + (double)first * tc.dt;
+        const double t_ref = src.t0 + (double)T * tc.dt;
         const double L_ref = orbit_val(orbit_node<EXACT>(orb, t_ref), rm.row[0]);
-        lti_reference<NX>(t_ref, first, L_ref, t0_offset, tc.fs, lref);
+        lti_reference<NX>(t_ref, T, L_ref, t0_offset, tc.fs, lref);
         lti = lref.ok;
         if (lti && tid < 2 * kFir) fir[tid] = lti_filter_entry<NX>(lref, lg, tid);
         __syncthreads();

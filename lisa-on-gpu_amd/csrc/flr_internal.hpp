@@ -39,13 +39,11 @@ struct LagrangeView {
 };
 
 // ------------------------------------------------------------------ projection
-// Shared by every source of a launch. Each spacecraft s (0..2) receives two links;
-// rec_row[s][q] is the table row (and y row) of the q-th link received by s, em_sc[s][q] its
-// emitter spacecraft.
+// Shared by every source of a launch. link_row[s][e] is the table row (and y row) of the link
+// received by spacecraft s (0..2) from spacecraft e (-1 for s == e).
 struct ProjConfig {
     double dt, fs;
-    int rec_row[3][2];
-    int em_sc[3][2];
+    int link_row[3][3];
     int halo_lo, halo_hi;   // staged window = [first - halo_lo, last + halo_hi]
     int win_cap;            // LDS window capacity (complex samples)
     int zero_edges;
@@ -151,10 +149,11 @@ struct FusedSource {
 
 // Launchers (flr_kernels.hip).
 hipError_t launch_projection(const ProjConfig &cfg, const ProjSource &src, const OrbitView &orb,
-                             const LagrangeView &lg, bool exact, hipStream_t stream);
+                             const LagrangeView &lg, bool exact, bool equal_arm,
+                             hipStream_t stream);
 hipError_t launch_projection_batch(const ProjConfig &cfg, const Batch<ProjSource> &batch,
                                    const OrbitView &orb, const LagrangeView &lg, bool exact,
-                                   int grid_per_cu, hipStream_t stream);
+                                   bool equal_arm, int grid_per_cu, hipStream_t stream);
 hipError_t launch_tdi(const TdiConfig &cfg, const TdiSource &src, const OrbitView &orb,
                       const LagrangeView &lg, bool exact, const StockTdi &stock,
                       hipStream_t stream);

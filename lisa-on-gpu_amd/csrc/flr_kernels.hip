@@ -8,7 +8,7 @@
 //   and 6 emitter stencils of every lane (the reference restages it per link,
 //   LISAResponse.cu:628-655). A receiver stencil depends on the receiving spacecraft only, so
 //   each is computed once for its two links: 9 complex stencils per sample instead of 12,
-//   with bit-identical results.
+//   with bit-identical results (6 with equal arms, where the emitter stencils are shared too).
 // tdi_tile: any TDI table. The 6 y rows of the window are staged once; each lane walks every
 //   unit of every channel for its sample, summing units of a channel in index order in
 //   registers (no atomics), then stores X, Y, Z or A, E, T.
@@ -23,7 +23,7 @@ namespace flr {
 
 // ------------------------------------------------------------------ projection
 
-template <bool EXACT, int H>
+template <bool EXACT, int H, bool EQ>
 __device__ __forceinline__ void projection_tile(const ProjConfig &cfg, const ProjSource &src,
                                                 int tile, const OrbitView &orb,
                                                 const LagrangeView &lg, double2 *hwin)
@@ -61,7 +61,7 @@ __device__ __forceinline__ void projection_tile(const ProjConfig &cfg, const Pro
     const double y_deps = EXACT ? rcp_nr2(lg.deps) : 0.0;
     double *y = src.y + i;
     const size_t stride = (size_t)src.num_delays;
-    if (project_links<EXACT, H>(cfg, src.k, src.u, src.v, t, t0_offset, orb, lg, hwin, win_lo,
+    if (project_links<EXACT, H, EQ>(cfg, src.k, src.u, src.v, t, t0_offset, orb, lg, hwin, win_lo,
                                 win_len, y_deps,
                                 [&](int row, double val) { y[row * stride] = val; }))
         atomicOr(cfg.err, kErrWindow);
@@ -191,21 +191,21 @@ __device__ __forceinline__ void tdi_equal_arm_tile(const TdiConfig &cfg, const T
 
 // ------------------------------------------------------------------ kernels
 
-template <bool EXACT, int H>
+template <bool EXACT, int H, bool EQ>
 __global__ __launch_bounds__(kBlock) void projection_kernel(ProjConfig cfg, ProjSource src,
                                                             OrbitView orb, LagrangeView lg)
 {
     extern __shared__ double2 hwin[];
-    projection_tile<EXACT, H>(cfg, src, blockIdx.x, orb, lg, hwin);
+    projection_tile<EXACT, H, EQ>(cfg, src, blockIdx.x, orb, lg, hwin);
 }
 
-template <bool EXACT, int H>
+template <bool EXACT, int H, bool EQ>
 __global__ __launch_bounds__(kBlock) void projection_batch_kernel(ProjConfig cfg,
                                                                   Batch<ProjSource> batch,
                                                                   OrbitView orb, LagrangeView lg)
 {
     extern __shared__ double2 hwin[];
-    FLR_BATCH_LOOP(batch, (projection_tile<EXACT, H>(cfg, src, tile, orb, lg, hwin)));
+    FLR_BATCH_LOOP(batch, (projection_tile<EXACT, H, EQ>(cfg, src, tile, orb, lg, hwin)));
 }
 
 template <bool EXACT, int H>
@@ -274,35 +274,44 @@ int batch_grid(int total_tiles, int grid_per_cu)
             cus <= 0)
             cus = 256;
     }
-    const int grid = cus * (grid_per_cu > 0 ? grid_per_cu : 8);
+    const int grid = cus * (grid_per_cu > 0 ? grid_per_cu : 16);
     return total_tiles < grid ? total_tiles : grid;
 }
 
 static size_t proj_lds(const ProjConfig &cfg) { return (size_t)cfg.win_cap * sizeof(double2); }
 
+// equal_arm: all six light travel time columns of the orbit table are bitwise equal (the
+// emitter stencils are then shared per emitting spacecraft; order 25 only).
 hipError_t launch_projection(const ProjConfig &cfg, const ProjSource &src, const OrbitView &orb,
-                             const LagrangeView &lg, bool exact, hipStream_t stream)
+                             const LagrangeView &lg, bool exact, bool equal_arm,
+                             hipStream_t stream)
 {
     const int nb = tiles_for(src.num_delays, src.start, cfg.zero_edges);
     const size_t shm = proj_lds(cfg);
+    if (lg.h == 13 && equal_arm)
+        return exact ? launch(projection_kernel<true, 13, true>, nb, shm, stream, cfg, src, orb, lg)
+                     : launch(projection_kernel<false, 13, true>, nb, shm, stream, cfg, src, orb, lg);
     if (lg.h == 13)
-        return exact ? launch(projection_kernel<true, 13>, nb, shm, stream, cfg, src, orb, lg)
-                     : launch(projection_kernel<false, 13>, nb, shm, stream, cfg, src, orb, lg);
-    return exact ? launch(projection_kernel<true, 0>, nb, shm, stream, cfg, src, orb, lg)
-                 : launch(projection_kernel<false, 0>, nb, shm, stream, cfg, src, orb, lg);
+        return exact ? launch(projection_kernel<true, 13, false>, nb, shm, stream, cfg, src, orb, lg)
+                     : launch(projection_kernel<false, 13, false>, nb, shm, stream, cfg, src, orb, lg);
+    return exact ? launch(projection_kernel<true, 0, false>, nb, shm, stream, cfg, src, orb, lg)
+                 : launch(projection_kernel<false, 0, false>, nb, shm, stream, cfg, src, orb, lg);
 }
 
 hipError_t launch_projection_batch(const ProjConfig &cfg, const Batch<ProjSource> &b,
                                    const OrbitView &orb, const LagrangeView &lg, bool exact,
-                                   int grid_per_cu, hipStream_t stream)
+                                   bool equal_arm, int grid_per_cu, hipStream_t stream)
 {
     const int nb = batch_grid(b.total_tiles, grid_per_cu);
     const size_t shm = proj_lds(cfg);
+    if (lg.h == 13 && equal_arm)
+        return exact ? launch(projection_batch_kernel<true, 13, true>, nb, shm, stream, cfg, b, orb, lg)
+                     : launch(projection_batch_kernel<false, 13, true>, nb, shm, stream, cfg, b, orb, lg);
     if (lg.h == 13)
-        return exact ? launch(projection_batch_kernel<true, 13>, nb, shm, stream, cfg, b, orb, lg)
-                     : launch(projection_batch_kernel<false, 13>, nb, shm, stream, cfg, b, orb, lg);
-    return exact ? launch(projection_batch_kernel<true, 0>, nb, shm, stream, cfg, b, orb, lg)
-                 : launch(projection_batch_kernel<false, 0>, nb, shm, stream, cfg, b, orb, lg);
+        return exact ? launch(projection_batch_kernel<true, 13, false>, nb, shm, stream, cfg, b, orb, lg)
+                     : launch(projection_batch_kernel<false, 13, false>, nb, shm, stream, cfg, b, orb, lg);
+    return exact ? launch(projection_batch_kernel<true, 0, false>, nb, shm, stream, cfg, b, orb, lg)
+                 : launch(projection_batch_kernel<false, 0, false>, nb, shm, stream, cfg, b, orb, lg);
 }
 
 static size_t tdi_lds(const TdiConfig &cfg, bool equal_arm)

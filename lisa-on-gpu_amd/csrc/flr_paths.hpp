@@ -34,7 +34,7 @@ __device__ __forceinline__ TileRange tile_range(int tile, int num_delays, int st
 // geometry and stencil; store(row, value) receives y of table row `row`. Operation order per
 // link follows LISAResponse.cu:559-668. Returns true if a stencil fell outside the window (its
 // links get NaN).
-template <bool EXACT, int H, typename Store>
+template <bool EXACT, int H, bool EQ, typename Store>
 __device__ __forceinline__ bool project_links(const ProjConfig &cfg, const double (&k)[3],
                                               const double (&u)[3], const double (&v)[3],
                                               double t, double t0_offset, const OrbitView &orb,
@@ -44,8 +44,28 @@ __device__ __forceinline__ bool project_links(const ProjConfig &cfg, const doubl
     const int h = H > 0 ? H : lg.h;
     const OrbitNode nr = orbit_node<EXACT>(orb, t);
     bool bad = false;
+    // EQ (all six light travel times bitwise equal): the emission time t - L, the emitter's
+    // position and so the emitter stencil depend on the emitting spacecraft only -- 3 emitter
+    // stencils instead of 6, each bit-identical to the per-link one.
+    double2 hem[3];
+    bool bem[3];
+    double xem[3][3];
+    if constexpr (EQ) {
+        const double t_em_eq = t - orbit_val(nr, cfg.link_row[0][1]);
+        const OrbitNode ne = orbit_node<EXACT>(orb, t_em_eq);
+        static_for<3>([&](auto se) {
 #pragma unroll
-    for (int s = 0; s < 3; ++s) {
+            for (int c = 0; c < 3; ++c) xem[se][c] = orbit_val(ne, 6 + 3 * se + c);
+            int d_em;
+            double e_em;
+            split_delay(t_em_eq - dot3(k, xem[se]) * kCInv, t0_offset, cfg.fs, d_em, e_em);
+            const int le = d_em - win_lo;
+            bem[se] = (le - (h - 1) < 0) || (le + h >= win_len);
+            hem[se] = make_double2(0.0, 0.0);
+            if (!bem[se]) hem[se] = stencil_c<EXACT, H>(hwin + le, e_em, lg, y_deps);
+        });
+    }
+    static_for<3>([&](auto s) {
         double xr[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) xr[c] = orbit_val(nr, 6 + 3 * s + c);
@@ -56,16 +76,29 @@ __device__ __forceinline__ bool project_links(const ProjConfig &cfg, const doubl
         const bool br = (lr - (h - 1) < 0) || (lr + h >= win_len);
         double2 hr = make_double2(0.0, 0.0);
         if (!br) hr = stencil_c<EXACT, H>(hwin + lr, e_rec, lg, y_deps);
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int row = cfg.rec_row[s][q];
-            const int se = cfg.em_sc[s][q];
-            const double L = orbit_val(nr, row);
-            const double t_em = t - L;
-            const OrbitNode ne = orbit_node<EXACT>(orb, t_em);
+        static_for<2>([&](auto q) {
+            constexpr int se = (s + 1 + q) % 3;  // the emitters of receiver s
+            const int row = cfg.link_row[s][se];
             double xe[3], n[3];
+            double2 he = make_double2(0.0, 0.0);
+            bool bq;
+            if constexpr (EQ) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) xe[c] = orbit_val(ne, 6 + 3 * se + c);
+                for (int c = 0; c < 3; ++c) xe[c] = xem[se][c];
+                bq = br || bem[se];
+                he = hem[se];
+            } else {
+                const double t_em = t - orbit_val(nr, row);
+                const OrbitNode ne = orbit_node<EXACT>(orb, t_em);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) xe[c] = orbit_val(ne, 6 + 3 * se + c);
+                int d_em;
+                double e_em;
+                split_delay(t_em - dot3(k, xe) * kCInv, t0_offset, cfg.fs, d_em, e_em);
+                const int le = d_em - win_lo;
+                bq = br || (le - (h - 1) < 0) || (le + h >= win_len);
+                if (!bq) he = stencil_c<EXACT, H>(hwin + le, e_em, lg, y_deps);
+            }
             double norm = 0.0;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
@@ -87,22 +120,16 @@ __device__ __forceinline__ bool project_links(const ProjConfig &cfg, const doubl
             const double xi_p = 0.5 * ((un * un) - (vn * vn));
             const double xi_c = un * vn;
             const double kn = dot3(k, n);
-            int d_em;
-            double e_em;
-            split_delay(t_em - dot3(k, xe) * kCInv, t0_offset, cfg.fs, d_em, e_em);
-            const int le = d_em - win_lo;
-            const bool bq = br || (le - (h - 1) < 0) || (le + h >= win_len);
             double out = __builtin_nan("");
             if (!bq) {
-                const double2 he = stencil_c<EXACT, H>(hwin + le, e_em, lg, y_deps);
                 const double pre = EXACT ? 1. / (1. - kn) : rcp_nr2(1. - kn);
                 const double large = (he.x - hr.x) * xi_p + (he.y - hr.y) * xi_c;
                 out = pre * large;
             }
             bad |= bq;
             store(row, out);
-        }
-    }
+        });
+    });
     return bad;
 }
 
@@ -192,7 +219,7 @@ inline hipError_t set_lds_limit(Kernel k, size_t bytes)
 
 // Persistent batch grid: grid_per_cu workgroups per CU (enough to fill every CU a few times
 // over), never more than there are tiles (flr_kernels.hip).
-int batch_grid(int total_tiles, int grid_per_cu = 8);
+int batch_grid(int total_tiles, int grid_per_cu = 16);
 
 template <typename Kernel, typename... Args>
 inline hipError_t launch(Kernel k, int nblocks, size_t shmem, hipStream_t stream, Args... args)

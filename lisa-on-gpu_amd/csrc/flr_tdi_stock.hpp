@@ -341,7 +341,7 @@ __device__ __forceinline__ bool tdi_linear(double t, double L, double t0_offset,
 // filter: the position of chain length m relative to sample i, delta_m = d_m - i + 1 - e_m
 // (the stencil interpolates at sample d_m + 1 - e_m), is the same for every i up to FP64
 // rounding of the times (~1e-9 samples). The groups of one series kind then add up to one
-// composite filter of at most kFir taps, built once per tile from the tile's first sample;
+// composite filter of at most kFir taps, built once per tile from a reference sample;
 // each sample checks that its own positions agree within kLtiTol samples (and falls back to
 // tdi_linear otherwise), so the result differs from per-sample weights by at most
 // kLtiTol * |dx/dsample| per stencil -- used by the mixed-precision kernels only.

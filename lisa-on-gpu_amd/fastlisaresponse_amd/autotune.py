@@ -32,7 +32,7 @@ def candidates(plan) -> Iterable[Tuple[int, int]]:
     else:
         tiles = (512,)
     for t in tiles:
-        for g in (4, 8, 16):
+        for g in (4, 8, 16, 32):
             yield (t, g)
 
 
