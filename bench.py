@@ -169,7 +169,9 @@ def batch_run(num_sources, steps, seed=3):
     del p64
 
     mb = BatchedResponse(1 / DT, tdi="2nd generation", orbits=orb, precision="mixed")
+    from fastlisaresponse_amd.batch import pack_q31
     h32, scales = pack_mixed(h, lengths)
+    hq, qscales = pack_q31(h, lengths)  # both forms quantise the same FP64 waveforms
     del h
     pm = mb.plan(h32, lengths, lam, beta, scales=scales)
     tune = pm.tune(use_cache=False, reps=2)
@@ -199,6 +201,28 @@ def batch_run(num_sources, steps, seed=3):
         sel = (np.log10(f0) >= lo) & (np.log10(f0) < lo + 1)
         by_f[f"f0 in [1e{lo}, 1e{lo + 1})"] = float(np.median(rel[sel, 0])) if sel.any() else None
     res["mixed_rel_l2_A_median_by_f0"] = by_f
+    del pm
+    # the same batch with 32-bit fixed-point input
+    qb = BatchedResponse(1 / DT, tdi="2nd generation", orbits=orb, precision="mixed-q31")
+    pq = qb.plan(hq, lengths, lam, beta, scales=qscales)
+    pq.tune(use_cache=False, reps=1)
+    wall, ms = time_plan(pq)
+    pq.check_errors()
+    res["q31_samples_per_s"] = total * steps / wall
+    relq = np.zeros((num_sources, 3))
+    o = 0
+    outq = pq.out_all.view(-1)
+    for s in range(num_sources):
+        n = int(lengths[s])
+        a = out64[3 * o: 3 * (o + n)].view(3, n)[:, T:n - T]
+        b = outq[3 * o: 3 * (o + n)].view(3, n)[:, T:n - T].double()
+        relq[s] = ((b - a).norm(dim=1) / a.norm(dim=1)).cpu().numpy()
+        o += n
+    res["q31_rel_l2_vs_fp64"] = {
+        ch: {"median": float(np.median(relq[:, c])), "p90": float(np.percentile(relq[:, c], 90)),
+             "max": float(relq[:, c].max()),
+             "sources_within_1e-6": int((relq[:, c] <= 1e-6).sum())}
+        for c, ch in enumerate("AET")}
     return res
 
 
@@ -317,6 +341,24 @@ def main() -> int:
             "hbm_achieved_GBs": BYTES_MIXED * N_SAMPLES / (ms * 1e-3) / 1e9,
             "hbm_frac": BYTES_MIXED * N_SAMPLES / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
+    out_q31 = None
+    if args.only is None:
+        from fastlisaresponse_amd.autotune import apply
+        from fastlisaresponse_amd.batch import pack_q31
+        bq = BatchedResponse(1 / DT, tdi=args.tdi, orbits=orbits, tdi_chan="AET",
+                             precision="mixed-q31")
+        hq, qscales = pack_q31(h, [N_SAMPLES])
+        pq = bq.plan(hq, [N_SAMPLES], [lam], [beta], scales=qscales)
+        shape = line["config"]["launch_shape"]
+        apply(pq, shape["tile_window"], shape["workgroups_per_cu"])
+        wq, msq = timed(pq.run, args.steps, args.warmup)
+        pq.check_errors()
+        out_q31 = pq.out_all.view(3, -1)
+        line["mixed_q31"] = {"value": N_SAMPLES * args.steps / wq,
+                             "ms_per_step": wq * 1e3 / args.steps, "kernel_ms": msq,
+                             "form": "same fused launch, input h / scale as 32-bit fixed point "
+                                     "(8 B/sample like complex64)"}
+
     if args.only in (None, "fp64"):
         r = pyResponseTDI(1 / DT, N_SAMPLES, order=ORDER, tdi=args.tdi, orbits=orbits,
                           tdi_chan="AET")
@@ -347,7 +389,7 @@ def main() -> int:
                                            x_base=orbits.x_base, nthreads=nth)
         par = {"vs": "FP64 CPU oracle, full workload"}
         T = int(T_BUFFER / DT)
-        for name, out in (("mixed", out_mixed), ("fp64", out_fp64)):
+        for name, out in (("mixed", out_mixed), ("mixed_q31", out_q31), ("fp64", out_fp64)):
             if out is None:
                 continue
             got = out.double().cpu().numpy()

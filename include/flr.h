@@ -39,6 +39,7 @@ extern "C" {
 #define FLR_FLAG_OVERWRITE 0x8  /* TDI: store the sums instead of adding them to the output */
 #define FLR_FLAG_AET 0x10       /* TDI: store A, E, T (implies OVERWRITE) instead of X, Y, Z */
 #define FLR_FLAG_MIXED 0x20     /* fused batches: complex64 input (h / scale), float32 output */
+#define FLR_FLAG_Q31 0x40       /* with FLR_FLAG_MIXED: input is int32 pairs, h / scale in 32-bit fixed point */
 
 typedef struct flr_orbits flr_orbits;
 typedef struct flr_response flr_response;
@@ -183,8 +184,9 @@ int flr_batch_create(flr_response *response, const flr_source *sources, int num_
  * by tile with the six projections kept on chip (y is never written to memory), for order 25.
  * FLR_FLAG_EXACT selects the exact stencils (FP64 only; then the outputs are bitwise those of
  * flr_batch_create + flr_batch_run), otherwise the fast form. With FLR_FLAG_MIXED, h_all holds
- * complex64 samples h / scale (scale per source) and out_all float32 outputs; without it h_all
- * is complex128 and out_all double. Outputs are overwritten (X, Y, Z, or A, E, T with
+ * complex64 samples h / scale (scale per source) and out_all float32 outputs; adding
+ * FLR_FLAG_Q31, h_all holds int32 pairs (re, im) = h / scale in fixed point (the same 8 bytes,
+ * 2^-31 full-scale steps); without FLR_FLAG_MIXED h_all is complex128 and out_all double. Outputs are overwritten (X, Y, Z, or A, E, T with
  * FLR_FLAG_AET); FLR_FLAG_ZERO_EDGES also zeroes samples outside [tdi_start_ind,
  * num_pts - tdi_start_ind). Run and destroy it with flr_batch_run / flr_batch_destroy. */
 int flr_fused_create(flr_response *response, const flr_source *sources, int num_sources,
@@ -208,7 +210,7 @@ int flr_batch_run(flr_batch *batch, void *stream);
  * workgroup tile (fused mixed: 256, 512 or 1024; fused FP64: 512; unfused: 256; 0 keeps the
  * current; a fused tile outputs tile_samples minus the TDI delay halos, so that its projection
  * window is whole rounds of the workgroup) and grid_per_cu persistent workgroups per CU
- * (1..64; 0 keeps the current, default 8). Results do not depend on either. Call it when no
+ * (1..64; 0 keeps the current, default 16). Results do not depend on either. Call it when no
  * launch of the batch is in flight; the autotuner (fastlisaresponse_amd.autotune) times the
  * candidates and keeps the fastest. */
 int flr_batch_set_launch(flr_batch *batch, int tile_samples, int grid_per_cu);

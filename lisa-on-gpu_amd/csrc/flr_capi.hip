@@ -862,6 +862,8 @@ int flr_fused_reserve(flr_response *r, int capacity, double dt, double sampling_
     if (r->order != 25) return fail(FLR_ERR_UNSUPPORTED, "fused batches are built for order 25");
     if ((flags & FLR_FLAG_MIXED) && (flags & FLR_FLAG_EXACT))
         return fail(FLR_ERR_UNSUPPORTED, "FLR_FLAG_EXACT needs FP64 storage (no FLR_FLAG_MIXED)");
+    if ((flags & FLR_FLAG_Q31) && !(flags & FLR_FLAG_MIXED))
+        return fail(FLR_ERR_INVALID_ARGUMENT, "FLR_FLAG_Q31 selects the mixed-precision input format");
     if (capacity < 1) return fail(FLR_ERR_INVALID_ARGUMENT, "capacity must be >= 1");
     if (!(sampling_frequency > 0.0) || !(dt > 0.0)) return fail(FLR_ERR_INVALID_ARGUMENT, "dt and sampling_frequency must be > 0");
     DeviceGuard g(r->device);
@@ -985,7 +987,8 @@ int flr_batch_run(flr_batch *b, void *stream)
             return st;
         Batch<FusedSource> bf{b->d_fused, b->d_fused_tiles, b->num_sources, b->fused_tiles};
         hipError_t e = launch_fused_batch(pc, tc, bf, r->orbits->view(), r->lagrange(), exact,
-                                          (b->flags & FLR_FLAG_MIXED) != 0, b->tile_passes,
+                                          (b->flags & FLR_FLAG_MIXED) != 0,
+                                          (b->flags & FLR_FLAG_Q31) != 0, b->tile_passes,
                                           b->grid_per_cu, stock, s);
         if (e != hipSuccess) return hip_fail(e, "fused batch launch");
         return FLR_OK;

@@ -136,7 +136,7 @@ inline int tiles_for(int num_delays, int start, int zero_edges, int tile = kBloc
 constexpr int kFusedPasses = 2;  // default tile: kFusedPasses * kBlock output samples
 
 struct FusedSource {
-    const void *h;          // [num_pts] complex: double2 (FP64), or float2 holding h / scale
+    const void *h;          // [num_pts] complex: double2 (FP64), float2 or int2 holding h / scale
     void *out;              // [3][num_pts] double (FP64) or float (mixed)
     int num_pts;
     int proj_start;         // projections_start_ind: y is zero outside [proj_start, n - proj_start)
@@ -160,13 +160,15 @@ hipError_t launch_tdi(const TdiConfig &cfg, const TdiSource &src, const OrbitVie
 hipError_t launch_tdi_batch(const TdiConfig &cfg, const Batch<TdiSource> &batch,
                             const OrbitView &orb, const LagrangeView &lg, bool exact,
                             const StockTdi &stock, int grid_per_cu, hipStream_t stream);
-// Fused batch: exact (FP64 only) or fast stencils; mixed = float2 h in, float out. Tiles of
+// Fused batch: exact (FP64 only) or fast stencils; mixed = float2 (or, with q31, int2
+// fixed-point) h in, float out. Tiles of
 // tile_passes * kBlock output samples (1, 2 or 4 for mixed, 2 for FP64); grid_per_cu
 // persistent workgroups per CU.
 hipError_t launch_fused_batch(const ProjConfig &pc, const TdiConfig &tc,
                               const Batch<FusedSource> &batch, const OrbitView &orb,
-                              const LagrangeView &lg, bool exact, bool mixed, int tile_passes,
-                              int grid_per_cu, const StockTdi &stock, hipStream_t stream);
+                              const LagrangeView &lg, bool exact, bool mixed, bool q31,
+                              int tile_passes, int grid_per_cu, const StockTdi &stock,
+                              hipStream_t stream);
 hipError_t launch_fused_fp64(const ProjConfig &pc, const TdiConfig &tc,
                              const Batch<FusedSource> &batch, const OrbitView &orb,
                              const LagrangeView &lg, bool exact, int grid_per_cu,
@@ -175,6 +177,10 @@ hipError_t launch_fused_mixed(const ProjConfig &pc, const TdiConfig &tc,
                               const Batch<FusedSource> &batch, const OrbitView &orb,
                               const LagrangeView &lg, int tile_passes, int grid_per_cu,
                               const StockTdi &stock, hipStream_t stream);
+hipError_t launch_fused_q31(const ProjConfig &pc, const TdiConfig &tc,
+                            const Batch<FusedSource> &batch, const OrbitView &orb,
+                            const LagrangeView &lg, int tile_passes, int grid_per_cu,
+                            const StockTdi &stock, hipStream_t stream);
 size_t fused_lds_bytes(const ProjConfig &pc, const TdiConfig &tc, bool generic);
 hipError_t launch_orbit_eval(const OrbitView &orb, const double *t, int n, int row, int sc,
                              double *ltt_out, double *pos_out, hipStream_t stream);

@@ -22,6 +22,7 @@ FLR_FLAG_ZERO_EDGES = 0x4
 FLR_FLAG_OVERWRITE = 0x8
 FLR_FLAG_AET = 0x10
 FLR_FLAG_MIXED = 0x20
+FLR_FLAG_Q31 = 0x40
 
 # name -> (restype, argtypes); kept in the order of include/flr.h
 _vp, _ci, _cd = ctypes.c_void_p, ctypes.c_int, ctypes.c_double

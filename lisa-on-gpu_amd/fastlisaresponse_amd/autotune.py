@@ -27,7 +27,7 @@ def candidates(plan) -> Iterable[Tuple[int, int]]:
     """(tile_samples, grid_per_cu) pairs valid for the plan."""
     if not plan.br.fused:
         tiles = (256,)
-    elif plan.br.precision == "mixed":
+    elif plan.br.precision != "fp64":
         tiles = (256, 512, 1024)
     else:
         tiles = (512,)

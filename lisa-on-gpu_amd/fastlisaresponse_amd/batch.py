@@ -15,7 +15,8 @@ tile results do not depend on their neighbours).
 Fused plans (fused=True, order 25) run projection and TDI as one launch with the projections
 kept in LDS; precision="mixed" additionally stores the input as complex64 h / scale (one scale
 per source, see pack_mixed) and the outputs as float32, with FP64 time/delay arithmetic and
-FP64 stencil arithmetic inside the kernel.
+FP64 stencil arithmetic inside the kernel; precision="mixed-q31" stores the input as 32-bit
+fixed point instead (pack_q31), ~2^7 times less quantisation error at the same 8 bytes.
 """
 from __future__ import annotations
 
@@ -53,24 +54,49 @@ def pack_mixed(h: torch.Tensor, lengths: Sequence[int]):
     return out, scales
 
 
+def pack_q31(h: torch.Tensor, lengths: Sequence[int]):
+    """complex128 packed sources -> (int32 [2 * samples] fixed point, scales): per source,
+    s = max(|Re h|, |Im h|) and each component is stored as round(x / s * (2^31 - 1)); the
+    returned scale is s / (2^31 - 1), so h = stored * scale. Same 8 bytes per sample as
+    complex64, with steps of 2^-31 of full scale instead of 2^-24 of the value."""
+    if not (isinstance(h, torch.Tensor) and h.dtype == torch.complex128):
+        raise TypeError("h must be a complex128 tensor")
+    lengths = [int(n) for n in lengths]
+    if sum(lengths) != h.numel():
+        raise ValueError("lengths do not add up to the number of samples")
+    full = float(2 ** 31 - 1)
+    out = torch.empty(2 * h.numel(), dtype=torch.int32, device=h.device)
+    scales = []
+    o = 0
+    for n in lengths:
+        seg = torch.view_as_real(h[o:o + n])
+        s = float(seg.abs().max()) if n else 1.0
+        s = s if s > 0.0 else 1.0
+        out[2 * o:2 * (o + n)] = torch.round(seg.reshape(-1) * (full / s)).to(torch.int32)
+        scales.append(s / full)
+        o += n
+    return out, scales
+
+
 class BatchedResponse:
     """Response configuration shared by a batch: sampling, order, TDI, orbits, channels.
 
     Args follow pyResponseTDI (sampling_frequency, order, tdi, orbits, tdi_chan, exact) plus
     t_buffer (s), the edge length left out of TDI for every source; fused (one launch, y kept
-    on chip, order 25) and precision ("fp64" or "mixed"; mixed implies fused and the fast
-    stencils). exact defaults to True for "fp64" and False for "mixed"."""
+    on chip, order 25) and precision ("fp64", "mixed" with complex64 input, or "mixed-q31"
+    with 32-bit fixed-point input, see pack_q31; both mixed forms imply fused and the fast
+    stencils). exact defaults to True for "fp64" and False otherwise."""
 
     def __init__(self, sampling_frequency: float, order: int = 25, tdi="2nd generation",
                  orbits=None, tdi_chan: str = "AET", exact: Optional[bool] = None,
                  t_buffer: float = 10000.0, fused: bool = False, precision: str = "fp64"):
         if tdi_chan not in ("XYZ", "AET", "AE"):
             raise ValueError("tdi_chan must be 'XYZ', 'AET' or 'AE'.")
-        if precision not in ("fp64", "mixed"):
-            raise ValueError("precision must be 'fp64' or 'mixed'")
+        if precision not in ("fp64", "mixed", "mixed-q31"):
+            raise ValueError("precision must be 'fp64', 'mixed' or 'mixed-q31'")
         if exact is None:
             exact = precision == "fp64"
-        if precision == "mixed":
+        if precision != "fp64":
             if exact:
                 raise ValueError("exact stencils need FP64 storage (precision='fp64')")
             fused = True
@@ -111,16 +137,19 @@ class BatchPlan:
             raise ValueError("lengths, lams, betas (and t0s, scales) must have one entry per source")
         if np.any(lengths < 1):
             raise ValueError("every source needs at least one sample")
-        mixed = br.precision == "mixed"
-        want = torch.complex64 if mixed else torch.complex128
+        mixed = br.precision != "fp64"
+        q31 = br.precision == "mixed-q31"
+        want = {"fp64": torch.complex128, "mixed": torch.complex64,
+                "mixed-q31": torch.int32}[br.precision]
         if not (isinstance(h_all, torch.Tensor) and h_all.is_cuda and h_all.dtype == want
                 and h_all.is_contiguous()):
             raise TypeError(f"h_all must be a contiguous {want} tensor on the GPU")
         if scales is not None and not br.fused:
             raise ValueError("scales apply to fused plans only")
         total = int(lengths.sum())
-        if h_all.numel() != total:
-            raise ValueError(f"h_all holds {h_all.numel()} samples, the lengths sum to {total}")
+        held = h_all.numel() // 2 if q31 else h_all.numel()
+        if held != total or (q31 and h_all.numel() % 2):
+            raise ValueError(f"h_all holds {held} samples, the lengths sum to {total}")
         self.br = br
         self.h_all = h_all
         self.lengths = lengths
@@ -148,6 +177,8 @@ class BatchPlan:
             flags |= _capi.FLR_FLAG_EXACT
         if mixed:
             flags |= _capi.FLR_FLAG_MIXED
+        if q31:
+            flags |= _capi.FLR_FLAG_Q31
         h = ctypes.c_void_p()
         if br.fused:
             check(lib().flr_fused_create(br.model.cpp_response._h, srcs, n, h_all.data_ptr(),

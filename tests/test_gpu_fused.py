@@ -7,7 +7,8 @@
   delays and stencils are FP64. It is pinned two ways: against the oracle run on the same
   complex64-rounded input (isolates the kernel: only the fast form's ~1e-15 and the float32
   output rounding remain, rel-L2 <= 2e-7 per channel), and against the oracle on the FP64
-  input (the storage error: rel-L2 <= 1e-6 on A and E at 5 mHz).
+  input (the storage error: rel-L2 <= 1e-6 on A and E at 5 mHz). The 32-bit fixed-point
+  input form ("mixed-q31") is pinned the same two ways.
 """
 import numpy as np
 import pytest
@@ -182,6 +183,54 @@ def test_mixed_vs_fp64_oracle_at_5mhz(torch_gpu, orbits, oracle_orbits, oracle_m
         assert rel_l2(y[l], y_exp[l]) <= TOL_FAST
 
 
+def test_q31_kernel_vs_oracle_on_dequantised_input(torch_gpu, orbits, oracle_orbits, oracle_mod):
+    """32-bit fixed-point input: the kernel against the oracle on the same quantised samples."""
+    torch = torch_gpu
+    from fastlisaresponse_amd.batch import BatchedResponse, pack_q31
+    rng = np.random.default_rng(47)
+    lengths = [50000, 13000]
+    hs, params = _sources(rng, lengths)
+    hq_all, scales = pack_q31(torch.as_tensor(np.concatenate(hs), device="cuda"), lengths)
+    assert hq_all.dtype == torch.int32 and hq_all.numel() == 2 * sum(lengths)
+    br = BatchedResponse(0.1, tdi="2nd generation", orbits=orbits, tdi_chan="AET",
+                         precision="mixed-q31")
+    plan = _plan(torch, br, hs, params, scales=scales, h_all=hq_all)
+    plan.run()
+    plan.check_errors()
+    q = hq_all.cpu().numpy().astype(np.float64).reshape(-1, 2)
+    o = 0
+    for s, n in enumerate(lengths):
+        lam, beta, t0 = params[s]
+        hq = (q[o:o + n, 0] + 1j * q[o:o + n, 1]) * scales[s]
+        o += n
+        _, exp = oracle_mod.response_pipeline(oracle_orbits, hq, lam, beta, 10.0, 25,
+                                              "2nd generation", t0=t0, x_base=orbits.x_base,
+                                              nthreads=8)
+        got = torch.stack(plan.channels(s)).cpu().numpy().astype(np.float64)
+        for c in range(3):
+            r = rel_l2(got[c, 1000:-1000], exp[c, 1000:-1000])
+            assert r <= TOL_MIXED_KERNEL, (s, c, r)
+
+
+def test_q31_vs_fp64_oracle_at_5mhz(torch_gpu, orbits, oracle_orbits, oracle_mod):
+    """pyResponseTDI(precision="mixed-q31"): storage error ~2^7 below complex64's."""
+    torch = torch_gpu
+    from fastlisaresponse_amd import pyResponseTDI
+    n = 1 << 18
+    h = gb_waveform(n, 10.0, 1e-21, 5e-3, 1e-16, 0.8, 0.3, 0.5)
+    r = pyResponseTDI(0.1, n, orbits=orbits, tdi="2nd generation", tdi_chan="AET",
+                      precision="mixed-q31")
+    r.get_projections(h, 1.2, 0.3)
+    got = torch.stack(r.get_tdi_delays()).cpu().numpy().astype(np.float64)
+    _, exp = oracle_mod.response_pipeline(oracle_orbits, h, 1.2, 0.3, 10.0, 25, "2nd generation",
+                                          x_base=orbits.x_base, nthreads=8)
+    # storage error predicted 2.8e-9, 2.4e-9, 1.3e-6 (oracle on the quantised input); on A and E
+    # the float32 output rounding (~3e-8 rms relative) dominates it
+    for c, tol in ((0, 1e-7), (1, 1e-7), (2, 1e-5)):
+        r_c = rel_l2(got[c, 1000:-1000], exp[c, 1000:-1000])
+        assert r_c <= tol, (c, r_c)
+
+
 def test_fused_argument_checks(torch_gpu, orbits):
     torch = torch_gpu
     from fastlisaresponse_amd import pyResponseTDI
@@ -201,7 +250,7 @@ def test_fused_argument_checks(torch_gpu, orbits):
         plan.projections(0)
 
 
-@pytest.mark.parametrize("kind", ["mixed", "fused-fp64", "unfused"])
+@pytest.mark.parametrize("kind", ["mixed", "mixed-q31", "fused-fp64", "unfused"])
 def test_launch_shapes_give_identical_results(torch_gpu, orbits, kind):
     """Tile length and grid density change speed only: every shape the autotuner may pick gives
     the same bits."""
@@ -211,17 +260,19 @@ def test_launch_shapes_give_identical_results(torch_gpu, orbits, kind):
     rng = np.random.default_rng(43)
     lengths = [30000, 4000, 12000]
     hs, params = _sources(rng, lengths)
-    kw = dict(mixed=dict(precision="mixed"), unfused=dict(),
-              **{"fused-fp64": dict(fused=True, exact=False)})[kind]
+    kw = {"mixed": dict(precision="mixed"), "mixed-q31": dict(precision="mixed-q31"),
+          "unfused": dict(), "fused-fp64": dict(fused=True, exact=False)}[kind]
     br = BatchedResponse(0.1, tdi="2nd generation", orbits=orbits, tdi_chan="AET", **kw)
     scales = None
     h_all = None
-    if kind == "mixed":
-        h_all, scales = pack_mixed(torch.as_tensor(np.concatenate(hs), device="cuda"), lengths)
+    if kind.startswith("mixed"):
+        from fastlisaresponse_amd.batch import pack_q31
+        pack = pack_q31 if kind == "mixed-q31" else pack_mixed
+        h_all, scales = pack(torch.as_tensor(np.concatenate(hs), device="cuda"), lengths)
     plan = _plan(torch, br, hs, params, scales=scales, h_all=h_all)
     ref = None
     shapes = list(candidates(plan))
-    assert len(shapes) == (9 if kind == "mixed" else 3)
+    assert len(shapes) == (12 if kind.startswith("mixed") else 4)
     for shape in shapes:
         apply(plan, *shape)
         plan.out_all.fill_(float("nan"))
