@@ -400,7 +400,7 @@ int make_fused_config(const flr_response *r, double dt, double fs, int flags, in
                                                                : passes * kBlock;
     tc.win_cap = tc.tile + halos;
     pc.win_cap = tc.win_cap + pc.halo_lo + pc.halo_hi;
-    const bool generic = !(stock.generation != 0 && stock.equal_arm);
+    const bool generic = stock.generation == 0 || r->order != 25;
     if (fused_lds_bytes(pc, tc, generic) > kMaxLds)
         return fail(FLR_ERR_UNSUPPORTED, "fused delay windows exceed LDS (sampling too fine)");
     return FLR_OK;

@@ -16,13 +16,15 @@ namespace flr {
 // The projected window overlaps the neighbouring tiles' by the TDI halo (~8% recompute at
 // 512-sample tiles and 2nd-generation equal arms).
 //
-// TDI modes: kModeGeneric -- any table, unit by unit (tdi_units); kModeGroups -- stock table
+// TDI modes: kModeGeneric -- any table, unit by unit (tdi_units); kModeStock -- stock table,
+// unequal arms, compile-time chains (tdi_stock_unequal); kModeGroups -- stock table
 // with equal arms, shared delays per chain length (tdi_equal_arm, exact or fast); kModeLinear --
 // the same, fast, with each group's units merged into one stencil per output (tdi_linear; the
 // window columns are replaced by the combined series right after projection).
 constexpr int kModeGeneric = 0;
 constexpr int kModeGroups = 1;
 constexpr int kModeLinear = 2;
+constexpr int kModeStock = 3;  // stock table, unequal arms: tdi_stock_unequal
 
 template <typename HT>
 __device__ __forceinline__ double2 load_h(const void *h, int j)
@@ -110,7 +112,7 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
             for (int r = 0; r < kNLinks; ++r) yc[r * cap] = 0.0;
         } else {
             const double t = src.t0 + (double)j * pc.dt;
-            bad |= project_links<EXACT, H, MODE != kModeGeneric>(pc, src.k, src.u, src.v, t, t0_offset, orb, lg, hwin,
+            bad |= project_links<EXACT, H, MODE == kModeGroups || MODE == kModeLinear>(pc, src.k, src.u, src.v, t, t0_offset, orb, lg, hwin,
                                            hw_lo, hw_len, y_deps,
                                            [&](int row, double val) { yc[row * cap] = val; });
         }
@@ -155,6 +157,16 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
             for (int q = 0; q < kNLinks; ++q) Ls[q * kBlock + tid] = orbit_val(nd, q);
             bad |= tdi_units<EXACT, H>(tc, t, t0_offset, Ls + tid, kBlock, ywin, cap, yw_lo,
                                        yw_len, lg, y_deps, acc);
+        } else if constexpr (MODE == kModeStock) {
+            const OrbitNode nd = orbit_node<EXACT>(orb, t);
+            double L[6];
+#pragma unroll
+            for (int q = 0; q < kNLinks; ++q) L[q] = orbit_val(nd, rm.row[q]);
+            int rowoff[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) rowoff[k] = rm.row[k] * cap - yw_lo;
+            bad |= !tdi_stock_unequal<NX, EXACT>(t, L, t0_offset, tc.fs, ywin, rowoff, yw_lo,
+                                                 yw_len, lg, y_deps, acc);
         } else {
             const double L = orbit_val(orbit_node<EXACT>(orb, t), rm.row[0]);
             if constexpr (MODE == kModeGroups) {
@@ -190,19 +202,16 @@ hipError_t launch_fused_variant(const ProjConfig &pc, const TdiConfig &tc, const
                                 int grid_per_cu, hipStream_t stream)
 {
     const int nb = batch_grid(b.total_tiles, grid_per_cu);
-    const bool eq = use_equal_arm(st, lg);
-    const size_t shm = fused_lds_bytes(pc, tc, !eq);
+    const size_t shm = fused_lds_bytes(pc, tc, !use_stock(st, lg));
     const RowMap &rm = st.rows;
     constexpr int kEq = EXACT ? kModeGroups : kModeLinear;
-    if (eq) {
-        if (st.generation == 1)
-            return launch(fused_batch_kernel<EXACT, kEq, 8, HT, OT, PASSES>, nb, shm, stream, pc, tc, b,
-                          orb, lg, rm);
-        return launch(fused_batch_kernel<EXACT, kEq, 16, HT, OT, PASSES>, nb, shm, stream, pc, tc, b,
-                      orb, lg, rm);
-    }
-    return launch(fused_batch_kernel<EXACT, kModeGeneric, 0, HT, OT, PASSES>, nb, shm, stream, pc, tc,
-                  b, orb, lg, rm);
+#define FLR_FUSED(M, NX) \
+    launch(fused_batch_kernel<EXACT, M, NX, HT, OT, PASSES>, nb, shm, stream, pc, tc, b, orb, lg, rm)
+    if (use_equal_arm(st, lg)) return st.generation == 1 ? FLR_FUSED(kEq, 8) : FLR_FUSED(kEq, 16);
+    if (use_stock(st, lg))
+        return st.generation == 1 ? FLR_FUSED(kModeStock, 8) : FLR_FUSED(kModeStock, 16);
+    return FLR_FUSED(kModeGeneric, 0);
+#undef FLR_FUSED
 }
 
 }  // namespace flr

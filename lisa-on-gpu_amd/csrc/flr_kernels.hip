@@ -12,7 +12,9 @@
 // tdi_tile: any TDI table. The 6 y rows of the window are staged once; each lane walks every
 //   unit of every channel for its sample, summing units of a channel in index order in
 //   registers (no atomics), then stores X, Y, Z or A, E, T.
-// tdi_equal_arm_tile: stock tables when all light travel times are equal (flr_tdi_stock.hpp).
+// tdi_stock_tile: the stock 1st/2nd-generation tables with compile-time chains
+//   (flr_tdi_stock.hpp): shared delays per chain length when all light travel times are
+//   equal, per-unit delays from registers otherwise.
 //
 // Each tile function is launched two ways: one workgroup per tile of a single source, or a
 // persistent batch kernel whose workgroups walk contiguous runs of a flattened (source, tile)
@@ -156,11 +158,11 @@ __device__ __forceinline__ void tdi_tile(const TdiConfig &cfg, const TdiSource &
 // (fast form) are shared by the units of that length -- 8 groups instead of 48 units for 2nd
 // generation, with bit-identical results.
 
-template <int NX, bool EXACT>
-__device__ __forceinline__ void tdi_equal_arm_tile(const TdiConfig &cfg, const TdiSource &src,
-                                                   int tile, const OrbitView &orb,
-                                                   const LagrangeView &lg, const RowMap &rm,
-                                                   double *ywin)
+template <int NX, bool EXACT, bool EQ>
+__device__ __forceinline__ void tdi_stock_tile(const TdiConfig &cfg, const TdiSource &src,
+                                               int tile, const OrbitView &orb,
+                                               const LagrangeView &lg, const RowMap &rm,
+                                               double *ywin)
 {
     const TileRange tr = tile_range(tile, src.num_delays, src.start, cfg.zero_edges);
     const int i = tr.i0 + (int)threadIdx.x;
@@ -171,7 +173,10 @@ __device__ __forceinline__ void tdi_equal_arm_tile(const TdiConfig &cfg, const T
     const int ic = valid ? i : tr.first;
     const double t = src.t_arr ? src.t_arr[ic] : src.t0 + (double)ic * cfg.dt;
     const double t0_offset = src.t_arr ? src.t_arr[0] : src.t0 + (double)0 * cfg.dt;
-    const double L = orbit_val(orbit_node<EXACT>(orb, t), rm.row[0]);
+    const OrbitNode nd = orbit_node<EXACT>(orb, t);
+    double L[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) L[k] = orbit_val(nd, rm.row[k]);
     int rowoff[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) rowoff[k] = rm.row[k] * cfg.win_cap - win_lo;
@@ -183,9 +188,14 @@ __device__ __forceinline__ void tdi_equal_arm_tile(const TdiConfig &cfg, const T
 #pragma unroll
     for (int c = 0; c < 3; ++c)
         acc[c] = cfg.overwrite ? 0.0 : src.out[(size_t)c * src.num_delays + i];
-    if (!tdi_equal_arm<NX, EXACT>(t, L, t0_offset, cfg.fs, ywin, rowoff, win_lo, win_len, lg,
-                                  y_deps, acc))
-        atomicOr(cfg.err, kErrWindow);
+    bool ok;
+    if constexpr (EQ)
+        ok = tdi_equal_arm<NX, EXACT>(t, L[0], t0_offset, cfg.fs, ywin, rowoff, win_lo, win_len, lg,
+                                      y_deps, acc);
+    else
+        ok = tdi_stock_unequal<NX, EXACT>(t, L, t0_offset, cfg.fs, ywin, rowoff, win_lo, win_len,
+                                          lg, y_deps, acc);
+    if (!ok) atomicOr(cfg.err, kErrWindow);
     tdi_store(cfg, src, i, acc);
 }
 
@@ -224,23 +234,23 @@ __global__ __launch_bounds__(kBlock) void tdi_batch_kernel(TdiConfig cfg, Batch<
     FLR_BATCH_LOOP(batch, (tdi_tile<EXACT, H>(cfg, src, tile, orb, lg, smem)));
 }
 
-template <int NX, bool EXACT>
-__global__ __launch_bounds__(kBlock) void tdi_equal_arm_kernel(TdiConfig cfg, TdiSource src,
+template <int NX, bool EXACT, bool EQ>
+__global__ __launch_bounds__(kBlock) void tdi_stock_kernel(TdiConfig cfg, TdiSource src,
                                                                OrbitView orb, LagrangeView lg,
                                                                RowMap rm)
 {
     extern __shared__ double smem[];
-    tdi_equal_arm_tile<NX, EXACT>(cfg, src, blockIdx.x, orb, lg, rm, smem);
+    tdi_stock_tile<NX, EXACT, EQ>(cfg, src, blockIdx.x, orb, lg, rm, smem);
 }
 
-template <int NX, bool EXACT>
-__global__ __launch_bounds__(kBlock) void tdi_equal_arm_batch_kernel(TdiConfig cfg,
+template <int NX, bool EXACT, bool EQ>
+__global__ __launch_bounds__(kBlock) void tdi_stock_batch_kernel(TdiConfig cfg,
                                                                      Batch<TdiSource> batch,
                                                                      OrbitView orb,
                                                                      LagrangeView lg, RowMap rm)
 {
     extern __shared__ double smem[];
-    FLR_BATCH_LOOP(batch, (tdi_equal_arm_tile<NX, EXACT>(cfg, src, tile, orb, lg, rm, smem)));
+    FLR_BATCH_LOOP(batch, (tdi_stock_tile<NX, EXACT, EQ>(cfg, src, tile, orb, lg, rm, smem)));
 }
 
 // ------------------------------------------------------------------ small kernels
@@ -314,23 +324,25 @@ hipError_t launch_projection_batch(const ProjConfig &cfg, const Batch<ProjSource
                  : launch(projection_batch_kernel<false, 0, false>, nb, shm, stream, cfg, b, orb, lg);
 }
 
-static size_t tdi_lds(const TdiConfig &cfg, bool equal_arm)
+// The stock kernels keep the light travel times in registers; the generic one in LDS.
+static size_t tdi_lds(const TdiConfig &cfg, bool stock)
 {
-    return ((size_t)kNLinks * cfg.win_cap + (equal_arm ? 0 : (size_t)kNLinks * kBlock)) * sizeof(double);
+    return ((size_t)kNLinks * cfg.win_cap + (stock ? 0 : (size_t)kNLinks * kBlock)) * sizeof(double);
 }
 
 hipError_t launch_tdi(const TdiConfig &cfg, const TdiSource &src, const OrbitView &orb,
                       const LagrangeView &lg, bool exact, const StockTdi &st, hipStream_t stream)
 {
     const int nb = tiles_for(src.num_delays, src.start, cfg.zero_edges);
-    if (use_equal_arm(st, lg)) {
+    if (use_stock(st, lg)) {
         const size_t shm = tdi_lds(cfg, true);
         const RowMap &rm = st.rows;
-        if (st.generation == 1)
-            return exact ? launch(tdi_equal_arm_kernel<8, true>, nb, shm, stream, cfg, src, orb, lg, rm)
-                         : launch(tdi_equal_arm_kernel<8, false>, nb, shm, stream, cfg, src, orb, lg, rm);
-        return exact ? launch(tdi_equal_arm_kernel<16, true>, nb, shm, stream, cfg, src, orb, lg, rm)
-                     : launch(tdi_equal_arm_kernel<16, false>, nb, shm, stream, cfg, src, orb, lg, rm);
+#define FLR_STOCK(NX, EQ)                                                                   \
+    (exact ? launch(tdi_stock_kernel<NX, true, EQ>, nb, shm, stream, cfg, src, orb, lg, rm) \
+           : launch(tdi_stock_kernel<NX, false, EQ>, nb, shm, stream, cfg, src, orb, lg, rm))
+        if (st.equal_arm) return st.generation == 1 ? FLR_STOCK(8, true) : FLR_STOCK(16, true);
+        return st.generation == 1 ? FLR_STOCK(8, false) : FLR_STOCK(16, false);
+#undef FLR_STOCK
     }
     const size_t shm = tdi_lds(cfg, false);
     if (lg.h == 13)
@@ -345,14 +357,15 @@ hipError_t launch_tdi_batch(const TdiConfig &cfg, const Batch<TdiSource> &b, con
                             int grid_per_cu, hipStream_t stream)
 {
     const int nb = batch_grid(b.total_tiles, grid_per_cu);
-    if (use_equal_arm(st, lg)) {
+    if (use_stock(st, lg)) {
         const size_t shm = tdi_lds(cfg, true);
         const RowMap &rm = st.rows;
-        if (st.generation == 1)
-            return exact ? launch(tdi_equal_arm_batch_kernel<8, true>, nb, shm, stream, cfg, b, orb, lg, rm)
-                         : launch(tdi_equal_arm_batch_kernel<8, false>, nb, shm, stream, cfg, b, orb, lg, rm);
-        return exact ? launch(tdi_equal_arm_batch_kernel<16, true>, nb, shm, stream, cfg, b, orb, lg, rm)
-                     : launch(tdi_equal_arm_batch_kernel<16, false>, nb, shm, stream, cfg, b, orb, lg, rm);
+#define FLR_STOCK(NX, EQ)                                                                        \
+    (exact ? launch(tdi_stock_batch_kernel<NX, true, EQ>, nb, shm, stream, cfg, b, orb, lg, rm)  \
+           : launch(tdi_stock_batch_kernel<NX, false, EQ>, nb, shm, stream, cfg, b, orb, lg, rm))
+        if (st.equal_arm) return st.generation == 1 ? FLR_STOCK(8, true) : FLR_STOCK(16, true);
+        return st.generation == 1 ? FLR_STOCK(8, false) : FLR_STOCK(16, false);
+#undef FLR_STOCK
     }
     const size_t shm = tdi_lds(cfg, false);
     if (lg.h == 13)

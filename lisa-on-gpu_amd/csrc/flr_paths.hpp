@@ -231,9 +231,16 @@ inline hipError_t launch(Kernel k, int nblocks, size_t shmem, hipStream_t stream
     return hipGetLastError();
 }
 
+// The stock-table kernels (compile-time chains, order 25); equal arms select their shared-delay
+// form.
+inline bool use_stock(const StockTdi &st, const LagrangeView &lg)
+{
+    return st.generation != 0 && lg.h == 13;
+}
+
 inline bool use_equal_arm(const StockTdi &st, const LagrangeView &lg)
 {
-    return st.generation != 0 && st.equal_arm && lg.h == 13;
+    return use_stock(st, lg) && st.equal_arm;
 }
 
 }  // namespace flr

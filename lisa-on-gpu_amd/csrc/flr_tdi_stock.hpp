@@ -244,6 +244,45 @@ __device__ __forceinline__ bool tdi_equal_arm(double t, double L, double t0_offs
     return ok;
 }
 
+// Stock TDI for one sample with arbitrary (unequal) light travel times L[k] (canonical link
+// order): the chains are compile-time, so each unit's delay is formed from registers in its
+// chain order -- the generic tdi_units' operation sequence unit by unit, hence bitwise equal to
+// it -- and every unit's stencil is addressed from its lowest tap. Units are added to their
+// channel in index order. Returns false if a stencil fell outside the window.
+template <int NX, bool EXACT>
+__device__ __forceinline__ bool tdi_stock_unequal(double t, const double (&L)[6], double t0_offset,
+                                                  double fs, const double *ywin,
+                                                  const int (&rowoff)[6], int win_lo, int win_len,
+                                                  const LagrangeView &lg, double y_deps,
+                                                  double (&acc)[3])
+{
+    constexpr int H = 13;
+    bool ok = true;
+    static_for<3>([&](auto c) {
+        static_for<NX>([&](auto x) {
+            // a fresh copy of t per unit: its delay arithmetic cannot be hoisted above the
+            // previous unit (which would keep every unit's split live at once)
+            double delay = t;
+            pin(delay);
+            static_for<stock::kX[x].len>([&](auto s) { delay -= L[stock::chain_of(c, x, s)]; });
+            int d;
+            double e;
+            split_delay(delay, t0_offset, fs, d, e);
+            const bool bad = (d - win_lo - (H - 1) < 0) || (d - win_lo + H >= win_len);
+            ok = ok && !bad;
+            int off = rowoff[stock::base_of(c, x)] + (bad ? win_lo + H - 1 : d) - (H - 1);
+            pin(off);
+            double val = stencil<EXACT, H>(ywin + off + (H - 1), e, lg, y_deps);
+            val = bad ? __builtin_nan("") : val;
+            val *= (double)stock::kX[x].sign;
+            acc[c] += val;
+            pin(acc[c]);
+            __asm__ volatile("" ::: "memory");
+        });
+    });
+    return ok;
+}
+
 // Column of the combined series (in place of the six y rows, same stride `cap`): row
 // 2*o + kind holds output o's combination of kind `kind` -- X, Y, Z themselves, or A, E, T
 // formed from them when aet is set. yc points at the column; rm maps canonical links to rows.

@@ -284,3 +284,49 @@ def test_launch_shapes_give_identical_results(torch_gpu, orbits, kind):
             assert torch.equal(plan.out_all, ref), shape
     res = plan.tune(use_cache=False, reps=1)
     assert tuple(res["best"]) in shapes and plan.launch_shape == tuple(res["best"])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_fused_window_error_is_reported_not_faulted(torch_gpu, orbits, precision):
+    """A source whose buffers are too short for the delays: the fused kernel's window checks
+    latch FLR_ERR_WINDOW and write NaN for the affected samples; nothing is read out of range
+    and the other samples are untouched by the failure."""
+    import ctypes
+    torch = torch_gpu
+    from fastlisaresponse_amd import _capi
+    from fastlisaresponse_amd._capi import FLRError, check, lib
+    from fastlisaresponse_amd.batch import BatchedResponse
+    from fastlisaresponse_amd.tables import sky_basis
+    br = BatchedResponse(0.1, orbits=orbits, precision=precision, fused=True,
+                         exact=False if precision == "fp64" else None)
+    n = 20000
+    t = np.arange(n) * 10.0
+    h = np.exp(2j * np.pi * 3e-3 * t)
+    hd = torch.as_tensor(h, device="cuda")
+    if precision == "mixed":
+        hd = hd.to(torch.complex64)
+    out = torch.zeros(3 * n, dtype=torch.float32 if precision == "mixed" else torch.float64,
+                      device="cuda")
+    srcs = (_capi.FlrSource * 1)()
+    q = srcs[0]
+    q.offset, q.num_pts, q.projections_start_ind, q.tdi_start_ind = 0, n, 0, 3
+    q.scale = 1.0
+    k, u, v = sky_basis(1.2, 0.3)
+    for c in range(3):
+        q.k[c], q.u[c], q.v[c] = k[c], u[c], v[c]
+    flags = _capi.FLR_FLAG_ZERO_EDGES | _capi.FLR_FLAG_AET
+    if precision == "mixed":
+        flags |= _capi.FLR_FLAG_MIXED
+    hb = ctypes.c_void_p()
+    check(lib().flr_fused_create(br.model.cpp_response._h, srcs, 1, hd.data_ptr(), out.data_ptr(),
+                                 br.dt, br.sampling_frequency, flags, ctypes.byref(hb)))
+    try:
+        check(lib().flr_batch_run(hb, None))
+        with pytest.raises(FLRError):
+            br.model.cpp_response.check_errors()
+    finally:
+        lib().flr_batch_destroy(hb)
+    o = out.view(3, n).double().cpu().numpy()
+    assert np.isnan(o[:, 3:40]).all()           # the first tile's samples: window error
+    assert np.isfinite(o[:, 5000:15000]).all()  # samples far from the ends are computed
+    br.model.cpp_response.check_errors()        # the error word was cleared
