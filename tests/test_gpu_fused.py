@@ -152,6 +152,36 @@ def test_mixed_kernel_vs_oracle_on_rounded_input(torch_gpu, orbits, oracle_orbit
             assert r <= TOL_MIXED_KERNEL, (s, c, r)
 
 
+@pytest.mark.parametrize("tdi,chan,dt", [("1st generation", "XYZ", 10.0),
+                                         ("2nd generation", "AET", 5.0),
+                                         ("1st generation", "AET", 5.0),
+                                         ("2nd generation", "XYZ", 2.0)])
+def test_mixed_generations_channels_and_sampling(torch_gpu, orbits, oracle_orbits, oracle_mod,
+                                                 tdi, chan, dt):
+    """Mixed kernels for both generations, both channel sets and sampling intervals where the
+    composite filter is nearly full (dt = 5 s: 38 of 40 taps) or does not fit (dt = 2 s: every
+    tile takes the per-sample path), against the oracle on the same complex64 input."""
+    torch = torch_gpu
+    from fastlisaresponse_amd.batch import BatchedResponse, pack_mixed
+    rng = np.random.default_rng(53)
+    n = 60000
+    f = 10 ** rng.uniform(-3, -2)
+    h = gb_waveform(n, dt, 1e-21, f, 1e-17, 0.7, 0.2, 0.4, t0=1.0e6)
+    h32, scales = pack_mixed(torch.as_tensor(h, device="cuda"), [n])
+    br = BatchedResponse(1.0 / dt, tdi=tdi, orbits=orbits, tdi_chan=chan, precision="mixed")
+    plan = br.plan(h32, [n], [0.9], [-0.4], [1.0e6], scales=scales)
+    plan.run()
+    plan.check_errors()
+    hq = h32.cpu().numpy().astype(np.complex128) * scales[0]
+    _, exp = oracle_mod.response_pipeline(oracle_orbits, hq, 0.9, -0.4, dt, 25, tdi, t0=1.0e6,
+                                          tdi_chan=chan, x_base=orbits.x_base, nthreads=8)
+    got = torch.stack(plan.channels(0)).cpu().numpy().astype(np.float64)
+    T = br.tdi_start_ind
+    for c in range(3):
+        r = rel_l2(got[c, T:-T], exp[c, T:-T])
+        assert r <= TOL_MIXED_KERNEL, (c, r)
+
+
 def test_mixed_vs_fp64_oracle_at_5mhz(torch_gpu, orbits, oracle_orbits, oracle_mod):
     """End to end through pyResponseTDI(precision="mixed"): the FP32 storage error."""
     torch = torch_gpu
