@@ -201,7 +201,7 @@ int set_lagrange_impl(flr_response *r, int order, const double *A_in, double dep
 }
 
 // True when the units, in channel-then-index order, are exactly the stock table of generation
-// `gen` (same links, signs, chains and channels), which tdi_equal_arm_kernel hard-codes.
+// `gen` (same links, signs, chains and channels), which the stock kernels hard-code.
 bool is_stock_table(int gen, const std::vector<int> &order_idx, const std::vector<int> &starts,
                     const std::vector<int> &lengths, const std::vector<int> &base,
                     const std::vector<int> &comb, const std::vector<double> &signs,

@@ -245,6 +245,9 @@ __device__ __forceinline__ double2 stencil_fast_c(const double2 *x, double e, co
         Tr = __builtin_fma(Tr, Q, nr * EP);
         Ti = __builtin_fma(Ti, Q, ni * EP);
         P = P * Q;
+        // one tap pair in registers at a time (the loads of the next j are not hoisted): keeps
+        // the fused kernels at 4 waves/SIMD
+        __asm__ volatile("" ::: "memory");
     }
     double invP = rcp_nr2(P);
     double2 x1 = x[1], x0 = x[0];

@@ -49,16 +49,17 @@ __device__ __forceinline__ bool project_links(const ProjConfig &cfg, const doubl
     // stencils instead of 6, each bit-identical to the per-link one.
     double2 hem[3];
     bool bem[3];
-    double xem[3][3];
+    OrbitNode ne_eq{};
     if constexpr (EQ) {
         const double t_em_eq = t - orbit_val(nr, cfg.link_row[0][1]);
-        const OrbitNode ne = orbit_node<EXACT>(orb, t_em_eq);
+        ne_eq = orbit_node<EXACT>(orb, t_em_eq);
         static_for<3>([&](auto se) {
+            double xe[3];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) xem[se][c] = orbit_val(ne, 6 + 3 * se + c);
+            for (int c = 0; c < 3; ++c) xe[c] = orbit_val(ne_eq, 6 + 3 * se + c);
             int d_em;
             double e_em;
-            split_delay(t_em_eq - dot3(k, xem[se]) * kCInv, t0_offset, cfg.fs, d_em, e_em);
+            split_delay(t_em_eq - dot3(k, xe) * kCInv, t0_offset, cfg.fs, d_em, e_em);
             const int le = d_em - win_lo;
             bem[se] = (le - (h - 1) < 0) || (le + h >= win_len);
             hem[se] = make_double2(0.0, 0.0);
@@ -82,9 +83,9 @@ __device__ __forceinline__ bool project_links(const ProjConfig &cfg, const doubl
             double xe[3], n[3];
             double2 he = make_double2(0.0, 0.0);
             bool bq;
-            if constexpr (EQ) {
+            if constexpr (EQ) {  // positions reloaded from the (cached) node: fewer live registers
 #pragma unroll
-                for (int c = 0; c < 3; ++c) xe[c] = xem[se][c];
+                for (int c = 0; c < 3; ++c) xe[c] = orbit_val(ne_eq, 6 + 3 * se + c);
                 bq = br || bem[se];
                 he = hem[se];
             } else {
