@@ -955,6 +955,7 @@ int flr_fused_create(flr_response *r, const flr_source *sources, int num_sources
 {
     if (!out) return fail(FLR_ERR_INVALID_ARGUMENT, "out is NULL");
     *out = nullptr;
+    if (!r || !r->orbits) return fail(FLR_ERR_NO_ORBITS, "Must add orbits with add_orbit_information method.");
     if (num_sources < 1 || !sources) return fail(FLR_ERR_INVALID_ARGUMENT, "no sources");
     flr_batch *b = nullptr;
     int st = flr_fused_reserve(r, num_sources, dt, sampling_frequency, flags, &b);

@@ -47,3 +47,30 @@ def test_host_only_entry_points():
                                  ctypes.byref(out)) == _capi.FLR_ERR_INVALID_ARGUMENT
     assert lib.flr_projection(None, None, None, 0, None, 0.0, 1.0, 1.0, None, None, None, 0, 0,
                               None) == _capi.FLR_ERR_INVALID_ARGUMENT
+
+
+def test_batch_entry_points_validate_before_device_work():
+    from fastlisaresponse_amd import _capi
+    lib = _capi.lib()
+    out = ctypes.c_void_p()
+    assert lib.flr_fused_reserve(None, 8, 10.0, 0.1, _capi.FLR_FLAG_MIXED,
+                                 ctypes.byref(out)) == _capi.FLR_ERR_NO_ORBITS
+    assert lib.flr_fused_create(None, None, 1, None, None, 10.0, 0.1, 0,
+                                ctypes.byref(out)) == _capi.FLR_ERR_NO_ORBITS
+    assert lib.flr_fused_bind(None, None, 1, None, None, None) == _capi.FLR_ERR_INVALID_ARGUMENT
+    assert lib.flr_batch_run(None, None) == _capi.FLR_ERR_INVALID_ARGUMENT
+    assert lib.flr_batch_set_launch(None, 512, 8) == _capi.FLR_ERR_INVALID_ARGUMENT
+    assert lib.flr_batch_create(None, None, 1, None, None, None, 10.0, 0.1, 0,
+                                ctypes.byref(out)) == _capi.FLR_ERR_NO_ORBITS
+    assert lib.flr_batch_samples(None) == 0
+    lib.flr_batch_destroy(None)  # accepts NULL
+    assert sizeof_source() == 112
+
+
+def sizeof_source():
+    """flr_source is 112 bytes in include/flr.h (int64, 4 x int32, t0, k, u, v, scale)."""
+    from fastlisaresponse_amd import _capi
+    s = _capi.FlrSource
+    assert s.offset.offset == 0 and s.num_pts.offset == 8 and s.t0.offset == 24
+    assert s.k.offset == 32 and s.u.offset == 56 and s.v.offset == 80 and s.scale.offset == 104
+    return ctypes.sizeof(s)

@@ -125,3 +125,68 @@ def test_poisson_traffic_mix():
     for r in reqs[:50]:
         assert len({s.num_pts for s in r.sources}) == 1
         assert all(-23 <= np.log10(s.scale) <= -21 for s in r.sources)
+
+
+def _shell_server(budget, max_sources=512, max_launch_samples=1 << 27, slots=2):
+    """A ResponseServer without device state: admission logic only (_launch recorded)."""
+    from collections import deque
+    from fastlisaresponse_amd.serving import ResponseServer
+    srv = object.__new__(ResponseServer)
+    srv.pending = deque()
+    srv.hbm_budget = budget
+    srv.in_flight_bytes = 0
+    srv.max_sources = max_sources
+    srv.max_launch_samples = max_launch_samples
+    srv.busy = [None] * slots
+    srv.launched = []
+
+    def launch(slot, reqs):
+        srv.launched.append((slot, [r.rid for r in reqs]))
+        srv.in_flight_bytes += sum(r.samples for r in reqs) * ResponseServer.BYTES_PER_SAMPLE
+        srv.busy[slot] = object()
+    srv._launch = launch
+    return srv
+
+
+def _req(rid, key, n, ns=1):
+    import torch
+    from fastlisaresponse_amd.serving import Request, SourceSpec
+    srcs = [SourceSpec(h=torch.zeros(n, dtype=torch.complex64), lam=0.0, beta=0.0) for _ in range(ns)]
+    return Request(rid, key[0], key[1], srcs)
+
+
+def test_serving_admission_merges_same_key_and_keeps_order():
+    from fastlisaresponse_amd.serving import ResponseServer
+    B = ResponseServer.BYTES_PER_SAMPLE
+    a, b = (10.0, "2nd generation"), (5.0, "1st generation")
+    srv = _shell_server(budget=10_000 * B, max_launch_samples=2500)
+    for r in [_req(0, a, 1000), _req(1, b, 1000), _req(2, a, 1000), _req(3, a, 1000),
+              _req(4, b, 500)]:
+        srv.pending.append(r)
+    assert srv.admit() == 2
+    # slot 0: head key a, merged up to 2500 samples (requests 0, 2; 3 would exceed the cap)
+    # slot 1: the next head (1, key b) merged with 4
+    assert srv.launched == [(0, [0, 2]), (1, [1, 4])]
+    assert [r.rid for r in srv.pending] == [3]
+    assert srv.in_flight_bytes == 3500 * B
+
+
+def test_serving_admission_respects_budget_and_head_of_line():
+    from fastlisaresponse_amd.serving import ResponseServer
+    B = ResponseServer.BYTES_PER_SAMPLE
+    a = (10.0, "2nd generation")
+    srv = _shell_server(budget=2500 * B, slots=4)
+    for r in [_req(0, a, 2000), _req(1, a, 1000), _req(2, a, 400)]:
+        srv.pending.append(r)
+    srv.admit()
+    # 0 is admitted; 1 does not fit beside it but 2 does, and merges with the head
+    assert srv.launched == [(0, [0, 2])]
+    assert srv.in_flight_bytes <= srv.hbm_budget
+    srv.admit()  # the head (1) no longer fits the remaining budget: nothing is admitted
+    assert srv.launched == [(0, [0, 2])] and [r.rid for r in srv.pending] == [1]
+    # the source cap splits a launch too
+    srv = _shell_server(budget=10 ** 9, max_sources=70, slots=1)
+    for r in [_req(0, a, 100, ns=40), _req(1, a, 100, ns=40)]:
+        srv.pending.append(r)
+    srv.admit()
+    assert srv.launched == [(0, [0])]
