@@ -218,6 +218,33 @@ int flr_batch_set_launch(flr_batch *batch, int tile_samples, int grid_per_cu);
 int64_t flr_batch_samples(const flr_batch *batch);
 void flr_batch_destroy(flr_batch *batch);
 
+/* ------------------------------------------------------------------ waveforms
+ * No counterpart in the reference library (its tests generate galactic binaries with numpy,
+ * tests/test_fastlisaresponse.py:41-63): the same generator on the device, writing straight into
+ * a packed response input. One record per source, in DEVICE memory, sorted by offset. */
+typedef struct {
+    int64_t offset;       /* first sample of the source in the packed output */
+    int32_t num_pts;
+    int32_t reserved;
+    double amplitude;     /* A */
+    double f0, fdot, fddot; /* Hz, Hz/s, Hz/s^2 */
+    double cos_iota;      /* cos(inclination) */
+    double cos2psi, sin2psi; /* polarisation rotation */
+    double phi0;          /* phase at t = 0 */
+    double t0;            /* time of the first sample (s) */
+    double scale;         /* complex64 / q31 formats store h / scale (q31: times 2^31 - 1) */
+} flr_gb_source;
+
+#define FLR_GB_COMPLEX128 0  /* h */
+#define FLR_GB_COMPLEX64 1   /* h / scale as complex64 */
+#define FLR_GB_Q31 2         /* h / scale as int32 pairs, full scale 2^31 - 1 */
+
+/* Generate num_sources galactic binaries (t = t0 + k*dt, k < num_pts) into `out` (packed, of
+ * total_samples entries in `format`) on `stream`. Allocates nothing; samples not covered by a
+ * source are left untouched. */
+int flr_gb_generate(const flr_gb_source *sources, int num_sources, int64_t total_samples,
+                    double dt, int format, void *out, void *stream);
+
 /* Stencil form used by the drop-in entry points flr_get_response / flr_get_tdi_delays:
  * nonzero (default) the reference operation order, bitwise equal to the FP64 CPU path;
  * 0 the faster reciprocal-free form (equal to ~1e-15 relative per stencil). */

@@ -6,7 +6,6 @@
 // the reference launchers do (LISAResponse.cu:420,704), and report latched device errors.
 #include "flr_internal.hpp"
 #include "flr_tdi_stock.hpp"
-#include "../../include/flr.h"
 
 #include <algorithm>
 #include <cmath>
@@ -1059,6 +1058,19 @@ int flr_batch_set_launch(flr_batch *b, int tile_samples, int grid_per_cu)
 }
 
 int64_t flr_batch_samples(const flr_batch *b) { return b ? b->samples : 0; }
+
+int flr_gb_generate(const flr_gb_source *sources, int num_sources, int64_t total_samples,
+                    double dt, int format, void *out, void *stream)
+{
+    if (!sources || num_sources < 1) return fail(FLR_ERR_INVALID_ARGUMENT, "no sources");
+    if (!out || total_samples < 1) return fail(FLR_ERR_INVALID_ARGUMENT, "empty output");
+    if (!(dt > 0.0)) return fail(FLR_ERR_INVALID_ARGUMENT, "dt must be > 0");
+    if (format != FLR_GB_COMPLEX128 && format != FLR_GB_COMPLEX64 && format != FLR_GB_Q31)
+        return fail(FLR_ERR_INVALID_ARGUMENT, "unknown waveform format");
+    hipError_t e = launch_gb(sources, num_sources, total_samples, dt, format, out, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "waveform launch");
+    return FLR_OK;
+}
 
 int flr_response_set_exact(flr_response *r, int exact)
 {

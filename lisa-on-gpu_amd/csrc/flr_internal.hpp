@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "../../include/flr.h"
+
 namespace flr {
 
 constexpr double kCInv = 3.3356409519815204e-09;  // 1/c in s/m (LISAResponse.hh:7)
@@ -182,6 +184,9 @@ hipError_t launch_fused_q31(const ProjConfig &pc, const TdiConfig &tc,
                             const LagrangeView &lg, int tile_passes, int grid_per_cu,
                             const StockTdi &stock, hipStream_t stream);
 size_t fused_lds_bytes(const ProjConfig &pc, const TdiConfig &tc, bool generic);
+// Galactic-binary generator (flr_waveform.hip).
+hipError_t launch_gb(const flr_gb_source *sources, int num_sources, int64_t total, double dt,
+                     int format, void *out, hipStream_t stream);
 hipError_t launch_orbit_eval(const OrbitView &orb, const double *t, int n, int row, int sc,
                              double *ltt_out, double *pos_out, hipStream_t stream);
 hipError_t launch_clear(int *word, hipStream_t stream);

@@ -24,6 +24,10 @@ FLR_FLAG_AET = 0x10
 FLR_FLAG_MIXED = 0x20
 FLR_FLAG_Q31 = 0x40
 
+FLR_GB_COMPLEX128 = 0
+FLR_GB_COMPLEX64 = 1
+FLR_GB_Q31 = 2
+
 # name -> (restype, argtypes); kept in the order of include/flr.h
 _vp, _ci, _cd = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
 SIGNATURES = {
@@ -53,6 +57,7 @@ SIGNATURES = {
     "flr_batch_set_launch": (_ci, [_vp, _ci, _ci]),
     "flr_batch_samples": (ctypes.c_int64, [_vp]),
     "flr_batch_destroy": (None, [_vp]),
+    "flr_gb_generate": (_ci, [_vp, _ci, ctypes.c_int64, _cd, _ci, _vp, _vp]),
     "flr_response_set_exact": (_ci, [_vp, _ci]),
 }
 

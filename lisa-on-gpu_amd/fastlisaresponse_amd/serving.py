@@ -11,9 +11,10 @@ synchronises the device:
 * admission -- the head request is admitted when the bytes in flight plus its own (packed input
   and float32 outputs) fit the HBM budget and a stream is free; queued requests with the same
   key are merged into the same launch up to `max_launch_samples` / `max_sources`.
-* launch    -- the admitted inputs are packed into one arena (device copies, standing in for
-  the client's upload), the sources are bound (asynchronous table upload) and one fused
-  projection + TDI launch is enqueued on the stream, followed by an event.
+* launch    -- the admitted inputs are packed into one arena (device copies of the clients'
+  waveforms, or waveforms generated on the device from galactic-binary parameters), the
+  sources are bound (asynchronous table upload) and one fused projection + TDI launch is
+  enqueued on the stream, followed by an event.
 * completion -- `poll` checks the events; a finished launch completes all its requests
   (latency = completion - arrival) and releases its memory.
 
@@ -40,16 +41,20 @@ from .tables import sky_basis
 
 @dataclass
 class SourceSpec:
-    """One source of a request: input slice h (complex64 h / scale, on the device)."""
-    h: torch.Tensor
+    """One source of a request: either its input h (complex64 h / scale, on the device) or the
+    galactic-binary parameters gb = (A, f0, fdot, iota, phi0, psi) and length n, from which the
+    server generates the input on the device."""
+    h: Optional[torch.Tensor]
     lam: float
     beta: float
     scale: float = 1.0
     t0: float = 0.0
+    gb: Optional[Tuple[float, ...]] = None
+    n: int = 0
 
     @property
     def num_pts(self) -> int:
-        return int(self.h.numel())
+        return int(self.h.numel()) if self.h is not None else int(self.n)
 
 
 @dataclass
@@ -237,6 +242,7 @@ class ResponseServer:
         now = time.perf_counter()
         srcs = (_capi.FlrSource * nsrc)()
         start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        gen_params, gen_len, gen_off, gen_t0, gen_idx = [], [], [], [], []
         with torch.cuda.stream(stream):
             start.record(stream)
             arena = torch.empty(total, dtype=torch.complex64, device="cuda")
@@ -247,20 +253,33 @@ class ResponseServer:
                 req.offsets = []
                 for sp in req.sources:
                     n = sp.num_pts
-                    arena[o:o + n].copy_(sp.h, non_blocking=True)
                     s = srcs[q]
+                    if sp.h is not None:
+                        arena[o:o + n].copy_(sp.h, non_blocking=True)
+                        s.scale = sp.scale
+                    else:  # generated below, all of the launch's parameter sources at once
+                        gen_params.append(sp.gb)
+                        gen_len.append(n)
+                        gen_off.append(o)
+                        gen_t0.append(sp.t0)
+                        gen_idx.append(q)
                     s.offset = o
                     s.num_pts = n
                     s.projections_start_ind = br.projections_start_ind
                     s.tdi_start_ind = br.tdi_start_ind
                     s.t0 = sp.t0
-                    s.scale = sp.scale
                     k, u, v = sky_basis(sp.lam, sp.beta)
                     for c in range(3):
                         s.k[c], s.u[c], s.v[c] = k[c], u[c], v[c]
                     req.offsets.append(o)
                     o += n
                     q += 1
+            if gen_params:
+                from .waveforms import generate_gb
+                _, gscales = generate_gb(np.asarray(gen_params), gen_len, key[0], fmt="complex64",
+                                         t0s=gen_t0, out=arena, stream=stream, offsets=gen_off)
+                for q, sc in zip(gen_idx, gscales):
+                    srcs[q].scale = float(sc)
             b = self._batch(slot, key)
             check(lib().flr_fused_bind(b, srcs, nsrc, arena.data_ptr(), out.data_ptr(),
                                        ctypes.c_void_p(stream.cuda_stream)))
@@ -330,12 +349,15 @@ def gb_pool(num: int, n: int, dt: float, seed: int = 0) -> List[torch.Tensor]:
     return out
 
 
-def poisson_requests(rng: np.random.Generator, rate: float, duration: float, pool,
+def poisson_requests(rng: np.random.Generator, rate: float, duration: float, pool=None,
                      lengths=(1 << 18, 1 << 20, 1 << 22), dts=(5.0, 10.0),
                      gens=(("1st generation", 0.3), ("2nd generation", 0.7)), max_sources=64):
     """Arrival offsets (s) and requests: Poisson arrivals at `rate`/s over `duration` s, each a
     batch of 1..max_sources sources sharing one length, dt and TDI generation (probabilities
-    in `gens`); sky uniform on the sphere, log10 amplitude uniform in [-23, -21]."""
+    in `gens`); sky uniform on the sphere, log10 amplitude uniform in [-23, -21]. With a `pool`
+    of complex64 waveforms each source takes a slice of one; without, each source carries
+    galactic-binary parameters (f0 log-uniform in [1e-4, 1e-2] Hz, fdot log-uniform in
+    [1e-19, 1e-15] Hz/s, cos i, phi0, psi uniform) and the server generates it."""
     times, reqs = [], []
     t = rng.exponential(1.0 / rate)
     rid = 0
@@ -349,10 +371,17 @@ def poisson_requests(rng: np.random.Generator, rate: float, duration: float, poo
         ns = int(rng.integers(1, max_sources + 1))
         srcs = []
         for _ in range(ns):
-            h = pool[int(rng.integers(len(pool)))][:n]
-            srcs.append(SourceSpec(h=h, lam=float(rng.uniform(0, 2 * np.pi)),
-                                   beta=float(np.arcsin(rng.uniform(-1, 1))),
-                                   scale=float(10 ** rng.uniform(-23, -21))))
+            lam = float(rng.uniform(0, 2 * np.pi))
+            beta = float(np.arcsin(rng.uniform(-1, 1)))
+            amp = float(10 ** rng.uniform(-23, -21))
+            if pool is not None:
+                h = pool[int(rng.integers(len(pool)))][:n]
+                srcs.append(SourceSpec(h=h, lam=lam, beta=beta, scale=amp))
+            else:
+                gb = (amp, float(10 ** rng.uniform(-4, -2)), float(10 ** rng.uniform(-19, -15)),
+                      float(np.arccos(rng.uniform(-1, 1))), float(rng.uniform(0, 2 * np.pi)),
+                      float(rng.uniform(0, 2 * np.pi)))
+                srcs.append(SourceSpec(h=None, lam=lam, beta=beta, gb=gb, n=n))
         reqs.append(Request(rid, dt, tdi, srcs))
         times.append(t)
         rid += 1

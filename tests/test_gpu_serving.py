@@ -111,3 +111,28 @@ def test_served_source_vs_oracle(torch_gpu, orbits, oracle_mod):
         checked += 1
     assert checked == 4
     server.close()
+
+
+def test_parameter_requests_generated_on_device(torch_gpu, orbits):
+    """Requests that carry galactic-binary parameters: the server generates their inputs on the
+    device; each result equals the same request run alone (generator + plain mixed plan)."""
+    torch = torch_gpu
+    from fastlisaresponse_amd.batch import BatchedResponse
+    from fastlisaresponse_amd.serving import ResponseServer, poisson_requests, run_open_loop
+    from fastlisaresponse_amd.waveforms import generate_gb
+    rng = np.random.default_rng(71)
+    times, reqs = poisson_requests(rng, 400.0, 0.05, lengths=(1 << 14, 1 << 15), max_sources=8)
+    assert reqs and all(s.h is None for r in reqs for s in r.sources)
+    server = ResponseServer(orbits, KEYS, num_streams=3, keep_results=True)
+    run_open_loop(server, times, reqs)
+    for req in reqs:
+        params = np.array([s.gb for s in req.sources])
+        lengths = [s.num_pts for s in req.sources]
+        h32, scales = generate_gb(params, lengths, req.dt, fmt="complex64")
+        br = BatchedResponse(1.0 / req.dt, tdi=req.tdi, orbits=orbits, precision="mixed")
+        plan = br.plan(h32, lengths, [s.lam for s in req.sources], [s.beta for s in req.sources],
+                       scales=list(scales))
+        plan.run()
+        plan.check_errors()
+        assert torch.equal(req.result, plan.out_all), req.rid
+    server.close()
