@@ -101,6 +101,22 @@ def test_overlap_helper():
     a = [rng.normal(size=256) for _ in range(3)]
     assert abs(get_overlap(a, a) - 1.0) < 1e-12
     assert abs(get_overlap(a, [-x for x in a]) + 1.0) < 1e-12
+    # cos vs sin of one Fourier bin: rfft overlap -i, so real part 0 and modulus 1
+    n, k0 = 256, 17
+    t = np.arange(n)
+    c, s = np.cos(2 * np.pi * k0 * t / n), np.sin(2 * np.pi * k0 * t / n)
+    assert abs(get_overlap(c, s)) < 1e-12
+    assert abs(get_overlap(c, s, phase_maximize=True) - 1.0) < 1e-12
+    # brute-force DFT over bins 0..n/2 on tiny channels
+    b = [rng.normal(size=16) for _ in range(3)]
+    a16 = [x[:16] for x in a]
+    W = np.exp(-2j * np.pi * np.outer(np.arange(9), np.arange(16)) / 16)
+    exp = np.mean([np.vdot(W @ x, W @ y) / np.sqrt(np.vdot(W @ x, W @ x) * np.vdot(W @ y, W @ y))
+                   for x, y in zip(a16, b)])
+    assert abs(get_overlap(a16, b) - exp.real) < 1e-12
+    assert abs(get_overlap(np.stack(a16), np.stack(b), phase_maximize=True) - abs(exp)) < 1e-12
+    with pytest.raises(ValueError):
+        get_overlap(a16, b[:2])
 
 
 def test_poisson_traffic_mix():
