@@ -5,18 +5,27 @@ Workload: one galactic-binary source over one year, N = 3,153,600 samples at dt 
 (f0 = 5 mHz, fdot = 1e-16), order-25 Lagrange stencils, 2nd-generation TDI, A/E/T output,
 analytic equal-arm cartwheel orbits tabulated every 1e4 s. A step is one pass of the response
 over the source with its input already resident in HBM; `value` is AET samples/s of the
-mixed-precision form: complex64 input h / max|h|, one fused projection + TDI launch (the six
-projections stay in LDS), float32 A, E, T out; times, delays and stencil arithmetic in FP64.
+mixed-precision form (--precision, default mixed-c128): one fused projection + TDI launch (the
+six projections stay in LDS), times, delays and stencil arithmetic in FP64, composite filters
+for the equal-arm TDI chains, float32 A, E, T out. The default reads the complex128 input, the
+one format whose error stays within 1e-6 (rel-L2) on all three channels; the complex64
+(h / max|h|) and 32-bit fixed-point inputs are timed beside it with their errors. The launch
+reads a few percent of HBM bandwidth, so the input format hardly changes the speed.
 
 Also on the same JSON line:
   fp64          the plain FP64 drop-in (separate projection and TDI launches, exact stencils,
-                bitwise equal to the CPU oracle) timed in the same run, and mixed / fp64
+                bitwise equal to the CPU oracle) timed in the same run, and mixed / fp64;
+                fp64_fast and fp64_fused, the FP64 forms with division-free stencils (drop-in,
+                and one fused launch), and mixed / each
+  mixed_q31,    the same fused launch with a 32-bit fixed-point or a complex128 input (float32
+  mixed_c128    outputs): less or no storage error at the same arithmetic
   roofline      the fused mixed-precision kernel's algorithmic FP64 rate (stencil
                 multiply-adds, 3744 FLOP per 2nd-generation sample) against the 78.6 TFLOP/s
                 FP64 vector peak, its launch time from HIP events in the timed region, and HBM
                 bytes per launch from the committed rocprofv3 counter summary
   batch         the 512-source batch (durations U[0.05, 1] yr): FP64 exact vs autotuned
-                mixed-precision samples/s, and the mixed form's per-source rel-L2
+                mixed-precision samples/s (complex64, fixed-point and complex128 inputs), and
+                each form's per-source rel-L2 against the FP64 exact batch
   latency       p50/p99 of a short open-loop serving run: Poisson arrivals at 200 req/s of
                 1..64-source requests (2^18/2^20/2^22 samples, dt 5/10 s, 30/70 1st/2nd gen)
   parity        rel-L2 / max-abs of A, E, T against the FP64 oracle on the full workload
@@ -52,7 +61,10 @@ FLOP_PROJ = 2 * TAPS * 6 * 2 * 2
 FLOP_TDI = {"1st generation": 2 * TAPS * 24, "2nd generation": 2 * TAPS * 48}
 # Algorithmic HBM bytes per sample: h in, A, E, T out (y stays on chip in the fused kernel);
 # the FP64 drop-in also writes and reads the six projections.
-BYTES_MIXED = 8 + 12
+BYTES_IO = {"mixed": 8 + 12, "mixed-q31": 8 + 12, "mixed-c128": 16 + 12}
+FORMS = {"mixed": "complex64 input h / max|h|, float32 A, E, T",
+         "mixed-q31": "32-bit fixed-point input h / max(|Re h|, |Im h|), float32 A, E, T",
+         "mixed-c128": "complex128 input (no storage error), float32 A, E, T"}
 BYTES_FP64 = 16 + 48 + 48 + 24
 
 
@@ -168,11 +180,30 @@ def batch_run(num_sources, steps, seed=3):
     out64 = p64.out_all.view(-1)
     del p64
 
-    mb = BatchedResponse(1 / DT, tdi="2nd generation", orbits=orb, precision="mixed")
+    T = int(T_BUFFER / DT)
+
+    def rel_l2(out):
+        """Per-source rel-L2 of a float32 batch output against the FP64 exact batch."""
+        rel = np.zeros((num_sources, 3))
+        o = 0
+        for s in range(num_sources):
+            n = int(lengths[s])
+            a = out64[3 * o: 3 * (o + n)].view(3, n)[:, T:n - T]
+            b = out[3 * o: 3 * (o + n)].view(3, n)[:, T:n - T].double()
+            rel[s] = ((b - a).norm(dim=1) / a.norm(dim=1)).cpu().numpy()
+            o += n
+        return rel
+
+    def summary(rel):
+        return {ch: {"median": float(np.median(rel[:, c])),
+                     "p90": float(np.percentile(rel[:, c], 90)), "max": float(rel[:, c].max()),
+                     "sources_within_1e-6": int((rel[:, c] <= 1e-6).sum())}
+                for c, ch in enumerate("AET")}
+
     from fastlisaresponse_amd.batch import pack_q31
     h32, scales = pack_mixed(h, lengths)
     hq, qscales = pack_q31(h, lengths)  # both forms quantise the same FP64 waveforms
-    del h
+    mb = BatchedResponse(1 / DT, tdi="2nd generation", orbits=orb, precision="mixed")
     pm = mb.plan(h32, lengths, lam, beta, scales=scales)
     tune = pm.tune(use_cache=False, reps=2)
     res["mixed_tune"] = {"best": tune["best"], "ms": tune["ms"]}
@@ -180,28 +211,14 @@ def batch_run(num_sources, steps, seed=3):
     pm.check_errors()
     res["mixed_samples_per_s"] = total * steps / wall
     res["mixed_vs_fp64"] = res["mixed_samples_per_s"] / res["fp64_exact_samples_per_s"]
-    # per-source rel-L2 of the mixed form against the FP64 exact batch
-    T = int(T_BUFFER / DT)
-    rel = np.zeros((num_sources, 3))
-    o = 0
-    outm = pm.out_all.view(-1)
-    for s in range(num_sources):
-        n = int(lengths[s])
-        a = out64[3 * o: 3 * (o + n)].view(3, n)[:, T:n - T]
-        b = outm[3 * o: 3 * (o + n)].view(3, n)[:, T:n - T].double()
-        rel[s] = ((b - a).norm(dim=1) / a.norm(dim=1)).cpu().numpy()
-        o += n
-    res["mixed_rel_l2_vs_fp64"] = {
-        ch: {"median": float(np.median(rel[:, c])), "p90": float(np.percentile(rel[:, c], 90)),
-             "max": float(rel[:, c].max()),
-             "sources_within_1e-6": int((rel[:, c] <= 1e-6).sum())}
-        for c, ch in enumerate("AET")}
+    rel = rel_l2(pm.out_all.view(-1))
+    res["mixed_rel_l2_vs_fp64"] = summary(rel)
     by_f = {}
     for lo in (-4, -3):
         sel = (np.log10(f0) >= lo) & (np.log10(f0) < lo + 1)
         by_f[f"f0 in [1e{lo}, 1e{lo + 1})"] = float(np.median(rel[sel, 0])) if sel.any() else None
     res["mixed_rel_l2_A_median_by_f0"] = by_f
-    del pm
+    del pm, h32
     # the same batch with 32-bit fixed-point input
     qb = BatchedResponse(1 / DT, tdi="2nd generation", orbits=orb, precision="mixed-q31")
     pq = qb.plan(hq, lengths, lam, beta, scales=qscales)
@@ -209,20 +226,17 @@ def batch_run(num_sources, steps, seed=3):
     wall, ms = time_plan(pq)
     pq.check_errors()
     res["q31_samples_per_s"] = total * steps / wall
-    relq = np.zeros((num_sources, 3))
-    o = 0
-    outq = pq.out_all.view(-1)
-    for s in range(num_sources):
-        n = int(lengths[s])
-        a = out64[3 * o: 3 * (o + n)].view(3, n)[:, T:n - T]
-        b = outq[3 * o: 3 * (o + n)].view(3, n)[:, T:n - T].double()
-        relq[s] = ((b - a).norm(dim=1) / a.norm(dim=1)).cpu().numpy()
-        o += n
-    res["q31_rel_l2_vs_fp64"] = {
-        ch: {"median": float(np.median(relq[:, c])), "p90": float(np.percentile(relq[:, c], 90)),
-             "max": float(relq[:, c].max()),
-             "sources_within_1e-6": int((relq[:, c] <= 1e-6).sum())}
-        for c, ch in enumerate("AET")}
+    res["q31_rel_l2_vs_fp64"] = summary(rel_l2(pq.out_all.view(-1)))
+    del pq, hq
+    # and reading the complex128 input itself (no storage error)
+    cb = BatchedResponse(1 / DT, tdi="2nd generation", orbits=orb, precision="mixed-c128")
+    pc = cb.plan(h, lengths, lam, beta)
+    pc.tune(use_cache=False, reps=1)
+    wall, ms = time_plan(pc)
+    pc.check_errors()
+    res["c128_samples_per_s"] = total * steps / wall
+    res["c128_vs_fp64"] = res["c128_samples_per_s"] / res["fp64_exact_samples_per_s"]
+    res["c128_rel_l2_vs_fp64"] = summary(rel_l2(pc.out_all.view(-1)))
     return res
 
 
@@ -249,8 +263,8 @@ def serving_run(seconds, rate, seed=5):
     stats.update({"arrival_rate_per_s": rate, "arrival_seconds": seconds,
                   "workload": "Poisson arrivals, 1-64 sources/request, N in {2^18, 2^20, 2^22}, "
                               "dt in {5, 10} s, 30/70 1st/2nd gen, galactic-binary parameters "
-                              "generated on the device, mixed precision, 4 streams, "
-                              "64 GiB admission budget"})
+                              "generated on the device as complex128, fused kernel with "
+                              "float32 outputs, 4 streams, 64 GiB admission budget"})
     return stats
 
 
@@ -273,6 +287,10 @@ def main() -> int:
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--launch", default=None, metavar="TILExGRID",
                     help="fixed launch shape of the mixed plan (e.g. 512x32) instead of autotuning")
+    ap.add_argument("--warm-seconds", type=float, default=1.5,
+                    help="seconds of untimed launches before the first measurement")
+    ap.add_argument("--precision", choices=list(FORMS), default="mixed-c128",
+                    help="input format of the headline mixed-precision form")
     ap.add_argument("--only", choices=["mixed", "fp64"], default=None,
                     help="time one form only (for profiling)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -302,7 +320,7 @@ def main() -> int:
     flop = FLOP_PROJ + FLOP_TDI[args.tdi]
     line = {"metric": METRIC, "unit": "samples/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
-            "vs_baseline": None, "dtype": "f64", "storage": "f32 (h, A/E/T)",
+            "vs_baseline": None, "dtype": "f64", "storage": FORMS[args.precision],
             "data": "synthetic (one galactic binary, f0=5 mHz, fdot=1e-16; analytic "
                     + ("equal-arm" if args.orbits == "equal" else "unequal-arm") + " orbits)",
             "config": {"workload": f"1-yr order-25 {args.tdi} TDI AET, mixed precision fused "
@@ -310,55 +328,75 @@ def main() -> int:
                        "num_pts": N_SAMPLES, "dt": DT, "order": ORDER, "tdi": args.tdi,
                        "tdi_chan": "AET", "orbits": args.orbits + "-arm cartwheel"}}
 
-    out_mixed = out_fp64 = None
-    if args.only in (None, "mixed"):
+    from fastlisaresponse_amd.autotune import apply
+    from fastlisaresponse_amd.batch import pack_q31
+
+    warmed = []
+
+    def warm_up(fn):
+        """Bring the GPU to its working clocks (untimed launches) before the first leg."""
+        if warmed:
+            return
+        end = time.perf_counter() + args.warm_seconds
+        while time.perf_counter() < end:
+            for _ in range(20):
+                fn()
+            torch.cuda.synchronize()
+        warmed.append(True)
+
+    def mixed_leg(precision, launch=None, reps=3):
+        """One fused mixed-precision plan over the workload: autotuned (or at `launch`), then
+        timed. Returns (result dict, output [3, N])."""
         br = BatchedResponse(1 / DT, tdi=args.tdi, orbits=orbits, tdi_chan="AET",
-                             precision="mixed")
-        h32, scales = pack_mixed(h, [N_SAMPLES])
-        plan = br.plan(h32, [N_SAMPLES], [lam], [beta], scales=scales)
-        if args.launch:
-            from fastlisaresponse_amd.autotune import apply
-            tile, grid = (int(x) for x in args.launch.lower().split("x"))
-            apply(plan, tile, grid)
-            line["config"]["launch_shape"] = {"tile_window": tile, "workgroups_per_cu": grid}
+                             precision=precision)
+        if precision == "mixed":
+            hin, sc = pack_mixed(h, [N_SAMPLES])
+        elif precision == "mixed-q31":
+            hin, sc = pack_q31(h, [N_SAMPLES])
         else:
-            tune = plan.tune(use_cache=False, reps=3)
-            line["config"]["launch_shape"] = {"tile_window": tune["best"][0],
-                                              "workgroups_per_cu": tune["best"][1],
-                                              "autotune_ms": tune["ms"]}
+            hin, sc = h, None
+        plan = br.plan(hin, [N_SAMPLES], [lam], [beta], scales=sc)
+        warm_up(plan.run)
+        if launch:
+            apply(plan, *launch)
+            shape = {"tile_window": launch[0], "workgroups_per_cu": launch[1]}
+        else:
+            tune = plan.tune(use_cache=False, reps=reps)
+            shape = {"tile_window": tune["best"][0], "workgroups_per_cu": tune["best"][1],
+                     "autotune_ms": tune["ms"]}
         wall, ms = timed(plan.run, args.steps, args.warmup)
         plan.check_errors()
-        out_mixed = plan.out_all.view(3, -1)
-        line.update({"value": N_SAMPLES * args.steps / wall, "ms_per_step": wall * 1e3 / args.steps})
+        return ({"value": N_SAMPLES * args.steps / wall, "ms_per_step": wall * 1e3 / args.steps,
+                 "kernel_ms": ms, "launch_shape": shape, "form": FORMS[precision]},
+                plan.out_all.view(3, -1))
+
+    outs = {}
+    out_fp64 = None
+    if args.only in (None, "mixed"):
+        launch = tuple(int(x) for x in args.launch.lower().split("x")) if args.launch else None
+        head, outs[args.precision] = mixed_leg(args.precision, launch)
+        ms = head["kernel_ms"]
+        line["config"]["precision"] = args.precision
+        line["config"]["form"] = head["form"]
+        line["config"]["launch_shape"] = head["launch_shape"]
+        line.update({"value": head["value"], "ms_per_step": head["ms_per_step"]})
         achieved = flop * N_SAMPLES / (ms * 1e-3) / 1e12
-        tile = line["config"]["launch_shape"]["tile_window"]
-        traffic = pmc_traffic(f"fused_mixed_1yr_{'2nd' if args.tdi.startswith('2nd') else '1st'}"
+        tile = head["launch_shape"]["tile_window"]
+        traffic = pmc_traffic(f"fused_{args.precision}_1yr_{'2nd' if args.tdi.startswith('2nd') else '1st'}"
                               f"_gen_{args.orbits}_pmc.txt", f"tile{tile}")
+        nbytes = BYTES_IO[args.precision]
         line["roofline"] = {
             "bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
-            "kernel": "fused_batch_kernel (mixed)", "kernel_ms": ms,
-            "algorithmic_flop_per_sample": flop, "algorithmic_bytes_per_sample": BYTES_MIXED,
-            "hbm_achieved_GBs": BYTES_MIXED * N_SAMPLES / (ms * 1e-3) / 1e9,
-            "hbm_frac": BYTES_MIXED * N_SAMPLES / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+            "kernel": f"fused_batch_kernel ({args.precision})", "kernel_ms": ms,
+            "algorithmic_flop_per_sample": flop, "algorithmic_bytes_per_sample": nbytes,
+            "hbm_achieved_GBs": nbytes * N_SAMPLES / (ms * 1e-3) / 1e9,
+            "hbm_frac": nbytes * N_SAMPLES / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
-    out_q31 = None
     if args.only is None:
-        from fastlisaresponse_amd.autotune import apply
-        from fastlisaresponse_amd.batch import pack_q31
-        bq = BatchedResponse(1 / DT, tdi=args.tdi, orbits=orbits, tdi_chan="AET",
-                             precision="mixed-q31")
-        hq, qscales = pack_q31(h, [N_SAMPLES])
-        pq = bq.plan(hq, [N_SAMPLES], [lam], [beta], scales=qscales)
-        shape = line["config"]["launch_shape"]
-        apply(pq, shape["tile_window"], shape["workgroups_per_cu"])
-        wq, msq = timed(pq.run, args.steps, args.warmup)
-        pq.check_errors()
-        out_q31 = pq.out_all.view(3, -1)
-        line["mixed_q31"] = {"value": N_SAMPLES * args.steps / wq,
-                             "ms_per_step": wq * 1e3 / args.steps, "kernel_ms": msq,
-                             "form": "same fused launch, input h / scale as 32-bit fixed point "
-                                     "(8 B/sample like complex64)"}
+        for prec in FORMS:  # the other input formats, each autotuned
+            if prec != args.precision:
+                line[prec.replace("-", "_")], outs[prec] = mixed_leg(prec, reps=2)
 
     if args.only in (None, "fp64"):
         r = pyResponseTDI(1 / DT, N_SAMPLES, order=ORDER, tdi=args.tdi, orbits=orbits,
@@ -369,6 +407,7 @@ def main() -> int:
             r.get_projections(h, lam, beta, t0=0.0, t_buffer=T_BUFFER)
             res["out"] = r.get_tdi_delays()
 
+        warm_up(fp64_step)
         wall64, ms64 = timed(fp64_step, args.steps, args.warmup)
         r.check_errors()
         out_fp64 = torch.stack(res["out"])
@@ -382,6 +421,39 @@ def main() -> int:
         elif "value" in line:
             line["mixed_vs_fp64"] = line["value"] / fp64["value"]
 
+    out_fast = out_fused64 = None
+    if args.only is None:
+        # the other FP64 GPU forms (rel-L2 <= 1e-12 against the oracle rather than bitwise):
+        # the drop-in with division-free stencils, and one fused FP64 launch (y kept in LDS)
+        rf = pyResponseTDI(1 / DT, N_SAMPLES, order=ORDER, tdi=args.tdi, orbits=orbits,
+                           tdi_chan="AET", exact=False)
+        resf = {}
+
+        def fast_step():
+            rf.get_projections(h, lam, beta, t0=0.0, t_buffer=T_BUFFER)
+            resf["out"] = rf.get_tdi_delays()
+
+        wf, _ = timed(fast_step, args.steps, args.warmup)
+        rf.check_errors()
+        out_fast = torch.stack(resf["out"])
+        bf = BatchedResponse(1 / DT, tdi=args.tdi, orbits=orbits, tdi_chan="AET",
+                             precision="fp64", exact=False, fused=True)
+        pf = bf.plan(h, [N_SAMPLES], [lam], [beta])
+        tf = pf.tune(use_cache=False, reps=2)
+        wfu, msfu = timed(pf.run, args.steps, args.warmup)
+        pf.check_errors()
+        out_fused64 = pf.out_all.view(3, -1)
+        line["fp64_fast"] = {"value": N_SAMPLES * args.steps / wf, "ms_per_step": wf * 1e3 / args.steps,
+                             "form": "plain FP64 drop-in, division-free stencils"}
+        line["fp64_fused"] = {"value": N_SAMPLES * args.steps / wfu,
+                              "ms_per_step": wfu * 1e3 / args.steps, "kernel_ms": msfu,
+                              "launch_shape": list(tf["best"]),
+                              "achieved_TFLOPs": flop * N_SAMPLES / (msfu * 1e-3) / 1e12,
+                              "form": "one fused FP64 launch, division-free stencils"}
+        if "value" in line:
+            line["mixed_vs_fp64_fast"] = line["value"] / line["fp64_fast"]["value"]
+            line["mixed_vs_fp64_fused"] = line["value"] / line["fp64_fused"]["value"]
+
     if not args.no_parity:
         from oracle import oracle as orc
         oo = orc.OracleOrbits(*orbits.pycppdetector_args)
@@ -390,7 +462,9 @@ def main() -> int:
                                            x_base=orbits.x_base, nthreads=nth)
         par = {"vs": "FP64 CPU oracle, full workload"}
         T = int(T_BUFFER / DT)
-        for name, out in (("mixed", out_mixed), ("mixed_q31", out_q31), ("fp64", out_fp64)):
+        for name, out in (*((p.replace("-", "_"), o) for p, o in outs.items()),
+                          ("fp64", out_fp64),
+                          ("fp64_fast", out_fast), ("fp64_fused", out_fused64)):
             if out is None:
                 continue
             got = out.double().cpu().numpy()

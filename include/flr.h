@@ -40,6 +40,7 @@ extern "C" {
 #define FLR_FLAG_AET 0x10       /* TDI: store A, E, T (implies OVERWRITE) instead of X, Y, Z */
 #define FLR_FLAG_MIXED 0x20     /* fused batches: complex64 input (h / scale), float32 output */
 #define FLR_FLAG_Q31 0x40       /* with FLR_FLAG_MIXED: input is int32 pairs, h / scale in 32-bit fixed point */
+#define FLR_FLAG_C128 0x80      /* with FLR_FLAG_MIXED: input is complex128 h / scale (no storage error) */
 
 typedef struct flr_orbits flr_orbits;
 typedef struct flr_response flr_response;
@@ -186,7 +187,9 @@ int flr_batch_create(flr_response *response, const flr_source *sources, int num_
  * flr_batch_create + flr_batch_run), otherwise the fast form. With FLR_FLAG_MIXED, h_all holds
  * complex64 samples h / scale (scale per source) and out_all float32 outputs; adding
  * FLR_FLAG_Q31, h_all holds int32 pairs (re, im) = h / scale in fixed point (the same 8 bytes,
- * 2^-31 full-scale steps); without FLR_FLAG_MIXED h_all is complex128 and out_all double. Outputs are overwritten (X, Y, Z, or A, E, T with
+ * 2^-31 full-scale steps); adding FLR_FLAG_C128 instead, h_all holds complex128 h / scale
+ * (16 bytes, no storage error; outputs still float32); without FLR_FLAG_MIXED h_all is
+ * complex128 and out_all double. Outputs are overwritten (X, Y, Z, or A, E, T with
  * FLR_FLAG_AET); FLR_FLAG_ZERO_EDGES also zeroes samples outside [tdi_start_ind,
  * num_pts - tdi_start_ind). Run and destroy it with flr_batch_run / flr_batch_destroy. */
 int flr_fused_create(flr_response *response, const flr_source *sources, int num_sources,

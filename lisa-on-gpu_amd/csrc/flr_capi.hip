@@ -861,8 +861,10 @@ int flr_fused_reserve(flr_response *r, int capacity, double dt, double sampling_
     if (r->order != 25) return fail(FLR_ERR_UNSUPPORTED, "fused batches are built for order 25");
     if ((flags & FLR_FLAG_MIXED) && (flags & FLR_FLAG_EXACT))
         return fail(FLR_ERR_UNSUPPORTED, "FLR_FLAG_EXACT needs FP64 storage (no FLR_FLAG_MIXED)");
-    if ((flags & FLR_FLAG_Q31) && !(flags & FLR_FLAG_MIXED))
-        return fail(FLR_ERR_INVALID_ARGUMENT, "FLR_FLAG_Q31 selects the mixed-precision input format");
+    if ((flags & (FLR_FLAG_Q31 | FLR_FLAG_C128)) && !(flags & FLR_FLAG_MIXED))
+        return fail(FLR_ERR_INVALID_ARGUMENT, "FLR_FLAG_Q31 and FLR_FLAG_C128 select a mixed-precision input format");
+    if ((flags & FLR_FLAG_Q31) && (flags & FLR_FLAG_C128))
+        return fail(FLR_ERR_INVALID_ARGUMENT, "FLR_FLAG_Q31 and FLR_FLAG_C128 exclude each other");
     if (capacity < 1) return fail(FLR_ERR_INVALID_ARGUMENT, "capacity must be >= 1");
     if (!(sampling_frequency > 0.0) || !(dt > 0.0)) return fail(FLR_ERR_INVALID_ARGUMENT, "dt and sampling_frequency must be > 0");
     DeviceGuard g(r->device);
@@ -903,7 +905,8 @@ int flr_fused_bind(flr_batch *b, const flr_source *sources, int num_sources, con
     DeviceGuard g(b->r->device);
     const int flags = b->flags;
     const int zero_edges = (flags & FLR_FLAG_ZERO_EDGES) ? 1 : 0;
-    const size_t hsz = (flags & FLR_FLAG_MIXED) ? 2 * sizeof(float) : 2 * sizeof(double);
+    const size_t hsz = ((flags & FLR_FLAG_MIXED) && !(flags & FLR_FLAG_C128)) ? 2 * sizeof(float)
+                                                                               : 2 * sizeof(double);
     const size_t osz = (flags & FLR_FLAG_MIXED) ? sizeof(float) : sizeof(double);
     std::vector<FusedSource> fs(num_sources);
     std::vector<int> ft(num_sources + 1, 0);
@@ -988,7 +991,9 @@ int flr_batch_run(flr_batch *b, void *stream)
         Batch<FusedSource> bf{b->d_fused, b->d_fused_tiles, b->num_sources, b->fused_tiles};
         hipError_t e = launch_fused_batch(pc, tc, bf, r->orbits->view(), r->lagrange(), exact,
                                           (b->flags & FLR_FLAG_MIXED) != 0,
-                                          (b->flags & FLR_FLAG_Q31) != 0, b->tile_passes,
+                                          (b->flags & FLR_FLAG_Q31) ? kInQ31
+                                          : (b->flags & FLR_FLAG_C128) ? kInC128 : kInC64,
+                                          b->tile_passes,
                                           b->grid_per_cu, stock, s);
         if (e != hipSuccess) return hip_fail(e, "fused batch launch");
         return FLR_OK;

@@ -162,13 +162,18 @@ hipError_t launch_tdi(const TdiConfig &cfg, const TdiSource &src, const OrbitVie
 hipError_t launch_tdi_batch(const TdiConfig &cfg, const Batch<TdiSource> &batch,
                             const OrbitView &orb, const LagrangeView &lg, bool exact,
                             const StockTdi &stock, int grid_per_cu, hipStream_t stream);
-// Fused batch: exact (FP64 only) or fast stencils; mixed = float2 (or, with q31, int2
-// fixed-point) h in, float out. Tiles of
+// Input formats of the mixed fused kernels (h / scale per source): complex64, 32-bit fixed-point
+// pairs, complex128.
+constexpr int kInC64 = 0;
+constexpr int kInQ31 = 1;
+constexpr int kInC128 = 2;
+// Fused batch: exact (FP64 only) or fast stencils; mixed = h in one of the kIn* formats, float
+// out. Tiles of
 // tile_passes * kBlock output samples (1, 2 or 4 for mixed, 2 for FP64); grid_per_cu
 // persistent workgroups per CU.
 hipError_t launch_fused_batch(const ProjConfig &pc, const TdiConfig &tc,
                               const Batch<FusedSource> &batch, const OrbitView &orb,
-                              const LagrangeView &lg, bool exact, bool mixed, bool q31,
+                              const LagrangeView &lg, bool exact, bool mixed, int input,
                               int tile_passes, int grid_per_cu, const StockTdi &stock,
                               hipStream_t stream);
 hipError_t launch_fused_fp64(const ProjConfig &pc, const TdiConfig &tc,
@@ -183,6 +188,10 @@ hipError_t launch_fused_q31(const ProjConfig &pc, const TdiConfig &tc,
                             const Batch<FusedSource> &batch, const OrbitView &orb,
                             const LagrangeView &lg, int tile_passes, int grid_per_cu,
                             const StockTdi &stock, hipStream_t stream);
+hipError_t launch_fused_c128(const ProjConfig &pc, const TdiConfig &tc,
+                             const Batch<FusedSource> &batch, const OrbitView &orb,
+                             const LagrangeView &lg, int tile_passes, int grid_per_cu,
+                             const StockTdi &stock, hipStream_t stream);
 size_t fused_lds_bytes(const ProjConfig &pc, const TdiConfig &tc, bool generic);
 // Galactic-binary generator (flr_waveform.hip).
 hipError_t launch_gb(const flr_gb_source *sources, int num_sources, int64_t total, double dt,

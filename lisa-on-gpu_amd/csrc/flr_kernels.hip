@@ -399,13 +399,23 @@ size_t fused_lds_bytes(const ProjConfig &pc, const TdiConfig &tc, bool generic)
 
 hipError_t launch_fused_batch(const ProjConfig &pc, const TdiConfig &tc,
                               const Batch<FusedSource> &b, const OrbitView &orb,
-                              const LagrangeView &lg, bool exact, bool mixed, bool q31,
+                              const LagrangeView &lg, bool exact, bool mixed, int input,
                               int tile_passes, int grid_per_cu, const StockTdi &st,
                               hipStream_t stream)
 {
     if (lg.h != 13) return hipErrorInvalidValue;  // fused kernels are built for order 25
-    if (mixed && q31) return launch_fused_q31(pc, tc, b, orb, lg, tile_passes, grid_per_cu, st, stream);
-    if (mixed) return launch_fused_mixed(pc, tc, b, orb, lg, tile_passes, grid_per_cu, st, stream);
+    if (mixed) {
+        switch (input) {
+        case kInC64:
+            return launch_fused_mixed(pc, tc, b, orb, lg, tile_passes, grid_per_cu, st, stream);
+        case kInQ31:
+            return launch_fused_q31(pc, tc, b, orb, lg, tile_passes, grid_per_cu, st, stream);
+        case kInC128:
+            return launch_fused_c128(pc, tc, b, orb, lg, tile_passes, grid_per_cu, st, stream);
+        default:
+            return hipErrorInvalidValue;
+        }
+    }
     if (tile_passes != 2) return hipErrorInvalidValue;
     return launch_fused_fp64(pc, tc, b, orb, lg, exact, grid_per_cu, st, stream);
 }

@@ -18,6 +18,7 @@ LIB_DIR = os.path.join(PKG_DIR, "lib")
 LIB_PATH = os.path.join(LIB_DIR, "libflr_hip.so")
 
 SOURCES = ["flr_kernels.hip", "flr_fused_fp64.hip", "flr_fused_mixed.hip", "flr_fused_q31.hip",
+           "flr_fused_c128.hip",
            "flr_waveform.hip", "flr_capi.hip"]
 HEADERS = ["flr_internal.hpp", "flr_device.hpp", "flr_tdi_stock.hpp", "flr_paths.hpp",
            "flr_fused.hpp"]

@@ -23,6 +23,7 @@ FLR_FLAG_OVERWRITE = 0x8
 FLR_FLAG_AET = 0x10
 FLR_FLAG_MIXED = 0x20
 FLR_FLAG_Q31 = 0x40
+FLR_FLAG_C128 = 0x80
 
 FLR_GB_COMPLEX128 = 0
 FLR_GB_COMPLEX64 = 1

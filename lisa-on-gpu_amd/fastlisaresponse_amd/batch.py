@@ -16,7 +16,9 @@ Fused plans (fused=True, order 25) run projection and TDI as one launch with the
 kept in LDS; precision="mixed" additionally stores the input as complex64 h / scale (one scale
 per source, see pack_mixed) and the outputs as float32, with FP64 time/delay arithmetic and
 FP64 stencil arithmetic inside the kernel; precision="mixed-q31" stores the input as 32-bit
-fixed point instead (pack_q31), ~2^7 times less quantisation error at the same 8 bytes.
+fixed point instead (pack_q31), ~2^7 times less quantisation error at the same 8 bytes;
+precision="mixed-c128" reads the complex128 input itself (16 bytes, no storage error) and keeps
+the float32 outputs and the mixed arithmetic.
 """
 from __future__ import annotations
 
@@ -31,6 +33,8 @@ from ._capi import check, lib
 from .backend import current_stream
 from .response import pyResponseTDI
 from .tables import buffer_indices, sky_basis
+
+PRECISIONS = ("fp64", "mixed", "mixed-q31", "mixed-c128")
 
 
 def pack_mixed(h: torch.Tensor, lengths: Sequence[int]):
@@ -83,17 +87,18 @@ class BatchedResponse:
 
     Args follow pyResponseTDI (sampling_frequency, order, tdi, orbits, tdi_chan, exact) plus
     t_buffer (s), the edge length left out of TDI for every source; fused (one launch, y kept
-    on chip, order 25) and precision ("fp64", "mixed" with complex64 input, or "mixed-q31"
-    with 32-bit fixed-point input, see pack_q31; both mixed forms imply fused and the fast
-    stencils). exact defaults to True for "fp64" and False otherwise."""
+    on chip, order 25) and precision ("fp64", "mixed" with complex64 input, "mixed-q31"
+    with 32-bit fixed-point input, see pack_q31, or "mixed-c128" with complex128 input; the
+    mixed forms imply fused and the fast stencils and write float32 outputs). exact defaults to
+    True for "fp64" and False otherwise."""
 
     def __init__(self, sampling_frequency: float, order: int = 25, tdi="2nd generation",
                  orbits=None, tdi_chan: str = "AET", exact: Optional[bool] = None,
                  t_buffer: float = 10000.0, fused: bool = False, precision: str = "fp64"):
         if tdi_chan not in ("XYZ", "AET", "AE"):
             raise ValueError("tdi_chan must be 'XYZ', 'AET' or 'AE'.")
-        if precision not in ("fp64", "mixed", "mixed-q31"):
-            raise ValueError("precision must be 'fp64', 'mixed' or 'mixed-q31'")
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {PRECISIONS}")
         if exact is None:
             exact = precision == "fp64"
         if precision != "fp64":
@@ -121,8 +126,9 @@ class BatchedResponse:
     def plan(self, h_all: torch.Tensor, lengths: Sequence[int], lams: Sequence[float],
              betas: Sequence[float], t0s: Optional[Sequence[float]] = None,
              scales: Optional[Sequence[float]] = None) -> "BatchPlan":
-        """Bind a packed input (sources back to back in h_all: complex128, or complex64
-        h / scale for precision="mixed") and allocate the outputs."""
+        """Bind a packed input (sources back to back in h_all: complex128 for "fp64" and
+        "mixed-c128", complex64 h / scale for "mixed", int32 pairs for "mixed-q31") and allocate
+        the outputs."""
         return BatchPlan(self, h_all, lengths, lams, betas, t0s, scales)
 
 
@@ -140,7 +146,7 @@ class BatchPlan:
         mixed = br.precision != "fp64"
         q31 = br.precision == "mixed-q31"
         want = {"fp64": torch.complex128, "mixed": torch.complex64,
-                "mixed-q31": torch.int32}[br.precision]
+                "mixed-q31": torch.int32, "mixed-c128": torch.complex128}[br.precision]
         if not (isinstance(h_all, torch.Tensor) and h_all.is_cuda and h_all.dtype == want
                 and h_all.is_contiguous()):
             raise TypeError(f"h_all must be a contiguous {want} tensor on the GPU")
@@ -179,6 +185,8 @@ class BatchPlan:
             flags |= _capi.FLR_FLAG_MIXED
         if q31:
             flags |= _capi.FLR_FLAG_Q31
+        if br.precision == "mixed-c128":
+            flags |= _capi.FLR_FLAG_C128
         h = ctypes.c_void_p()
         if br.fused:
             check(lib().flr_fused_create(br.model.cpp_response._h, srcs, n, h_all.data_ptr(),

@@ -23,7 +23,8 @@ returned are torch tensors on the GPU. Differences, all deliberate:
   ~1e-7..1e-6 on A, E at mHz frequencies, growing as f^-3 below). `y_gw` is then computed
   on access, in FP64 from the original input. `precision="mixed-q31"` stores h as 32-bit
   fixed point instead (same 8 bytes per sample, ~2^7 times smaller storage error: A, E ~3e-9
-  and T ~1e-6 at 5 mHz).
+  and T ~1e-6 at 5 mHz); `precision="mixed-c128"` keeps the complex128 input itself (no
+  storage error: the float32 rounding of the outputs is what remains).
 """
 from __future__ import annotations
 
@@ -78,7 +79,8 @@ class pyResponseTDI:
         exact (bool): reference stencil operation order, bitwise FP64 CPU parity (default for
             precision "fp64"); False selects the faster reciprocal-free stencils.
         precision (str): "fp64" (default), "mixed" (complex64 input, float32 outputs, fused
-            launch, order 25) or "mixed-q31" (the same with 32-bit fixed-point input).
+            launch, order 25), "mixed-q31" (the same with 32-bit fixed-point input) or
+            "mixed-c128" (the same reading the complex128 input).
     """
 
     def __init__(self, sampling_frequency, num_pts, order=25, tdi="1st generation",
@@ -86,8 +88,8 @@ class pyResponseTDI:
                  tdi_chan="XYZ", force_backend=None, exact: Optional[bool] = None,
                  precision: str = "fp64"):
         _check_backend(force_backend)
-        if precision not in ("fp64", "mixed", "mixed-q31"):
-            raise ValueError("precision must be 'fp64', 'mixed' or 'mixed-q31'")
+        if precision not in ("fp64", "mixed", "mixed-q31", "mixed-c128"):
+            raise ValueError("precision must be 'fp64', 'mixed', 'mixed-q31' or 'mixed-c128'")
         if exact is None:
             exact = precision == "fp64"
         if precision != "fp64" and exact:
@@ -294,9 +296,13 @@ class pyResponseTDI:
         self._input_keepalive = h
         if self.precision != "fp64":
             from .batch import pack_mixed, pack_q31
-            pack = pack_q31 if self.precision == "mixed-q31" else pack_mixed
-            stored, scales = pack(h, [len(h)])
-            self._mixed = (stored, scales[0], k, u, v)
+            if self.precision == "mixed-c128":
+                stored, scale = h, 1.0
+            else:
+                pack = pack_q31 if self.precision == "mixed-q31" else pack_mixed
+                stored, scales = pack(h, [len(h)])
+                scale = scales[0]
+            self._mixed = (stored, scale, k, u, v)
             self.y_gw_flat = None
             self.y_gw_length = self.num_pts
             return
@@ -311,7 +317,7 @@ class pyResponseTDI:
         return self.delayed_links_flat.reshape(3, -1)
 
     def _run_fused(self, t0: float, aet: bool) -> torch.Tensor:
-        """precision="mixed": projection + TDI in one launch from the stored complex64 input."""
+        """Mixed precisions: projection + TDI in one launch from the stored input."""
         import ctypes
         h32, scale, k, u, v = self._mixed
         out = torch.empty(3 * self.num_pts, dtype=torch.float32, device=self.device)
@@ -328,6 +334,8 @@ class pyResponseTDI:
         flags = _capi.FLR_FLAG_ZERO_EDGES | _capi.FLR_FLAG_OVERWRITE | _capi.FLR_FLAG_MIXED
         if self.precision == "mixed-q31":
             flags |= _capi.FLR_FLAG_Q31
+        elif self.precision == "mixed-c128":
+            flags |= _capi.FLR_FLAG_C128
         if aet:
             flags |= _capi.FLR_FLAG_AET
         L = _capi.lib()

@@ -2,7 +2,7 @@
 
 A request is a batch of sources (e.g. the walkers of one MCMC step) that share a sampling
 interval and a TDI generation; each source has its own length, sky position, amplitude scale
-and complex64 input h / scale. The server keeps resident, per (dt, generation), one response
+and input h / scale (complex128 by default, or complex64 with precision="mixed"). The server keeps resident, per (dt, generation), one response
 handle (orbit, Lagrange and TDI tables on the device) and, per stream, one reserved fused batch
 (device source table + pinned staging), so the loop allocates no device tables and never
 synchronises the device:
@@ -41,7 +41,8 @@ from .tables import sky_basis
 
 @dataclass
 class SourceSpec:
-    """One source of a request: either its input h (complex64 h / scale, on the device) or the
+    """One source of a request: either its input h (h / scale on the device, in the server's
+    input format: complex128, or complex64 for precision="mixed") or the
     galactic-binary parameters gb = (A, f0, fdot, iota, phi0, psi) and length n, from which the
     server generates the input on the device."""
     h: Optional[torch.Tensor]
@@ -109,17 +110,26 @@ class ResponseServer:
         max_launch_samples, max_sources: caps on one merged launch.
         keep_results: keep each request's outputs (tests); otherwise they are dropped.
         t_buffer: edge length (s) left out of TDI, as pyResponseTDI.get_projections.
+        precision: "mixed-c128" (complex128 inputs, float32 outputs; default) or "mixed"
+            (complex64 inputs h / scale: half the input memory, FP32 storage error).
     """
 
-    BYTES_PER_SAMPLE = 8 + 12  # complex64 input + float32 A, E, T
+    INPUT_BYTES = {"mixed": 8, "mixed-c128": 16}
+    BYTES_PER_SAMPLE = 16 + 12  # per sample in flight: input + float32 A, E, T (default format)
 
     def __init__(self, orbits, keys: Sequence[Tuple[float, str]], hbm_budget: int = 64 << 30,
                  num_streams: int = 4, max_launch_samples: int = 1 << 27, max_sources: int = 512,
-                 keep_results: bool = False, t_buffer: float = 10000.0, tdi_chan: str = "AET"):
+                 keep_results: bool = False, t_buffer: float = 10000.0, tdi_chan: str = "AET",
+                 precision: str = "mixed-c128"):
+        if precision not in self.INPUT_BYTES:
+            raise ValueError(f"precision must be one of {sorted(self.INPUT_BYTES)}")
+        self.precision = precision
+        self.BYTES_PER_SAMPLE = self.INPUT_BYTES[precision] + 12
+        self._in_dtype = torch.complex128 if precision == "mixed-c128" else torch.complex64
         self.models: Dict[Tuple[float, str], BatchedResponse] = {}
         for dt, tdi in keys:
             self.models[(dt, tdi)] = BatchedResponse(1.0 / dt, tdi=tdi, orbits=orbits,
-                                                     tdi_chan=tdi_chan, precision="mixed",
+                                                     tdi_chan=tdi_chan, precision=precision,
                                                      t_buffer=t_buffer)
         self.hbm_budget = int(hbm_budget)
         self.max_launch_samples = int(max_launch_samples)
@@ -140,7 +150,8 @@ class ResponseServer:
         if b is None:
             br = self.models[key]
             flags = (_capi.FLR_FLAG_ZERO_EDGES | _capi.FLR_FLAG_OVERWRITE | _capi.FLR_FLAG_MIXED |
-                     (_capi.FLR_FLAG_AET if br.tdi_chan != "XYZ" else 0))
+                     (_capi.FLR_FLAG_AET if br.tdi_chan != "XYZ" else 0) |
+                     (_capi.FLR_FLAG_C128 if self.precision == "mixed-c128" else 0))
             b = ctypes.c_void_p()
             check(lib().flr_fused_reserve(br.model.cpp_response._h, self.max_sources, br.dt,
                                           br.sampling_frequency, flags, ctypes.byref(b)))
@@ -167,6 +178,8 @@ class ResponseServer:
             raise ValueError(f"no resident response for dt={req.dt}, tdi={req.tdi!r}")
         if not req.sources or len(req.sources) > self.max_sources:
             raise ValueError("a request needs 1..max_sources sources")
+        if any(sp.h is not None and sp.h.dtype != self._in_dtype for sp in req.sources):
+            raise TypeError(f"source inputs must be {self._in_dtype} for precision={self.precision!r}")
         need = req.samples * self.BYTES_PER_SAMPLE
         if need > self.hbm_budget:
             raise ValueError("request larger than the HBM budget")
@@ -245,7 +258,7 @@ class ResponseServer:
         gen_params, gen_len, gen_off, gen_t0, gen_idx = [], [], [], [], []
         with torch.cuda.stream(stream):
             start.record(stream)
-            arena = torch.empty(total, dtype=torch.complex64, device="cuda")
+            arena = torch.empty(total, dtype=self._in_dtype, device="cuda")
             out = torch.empty(3 * total, dtype=torch.float32, device="cuda")
             o = q = 0
             for req in reqs:
@@ -276,7 +289,8 @@ class ResponseServer:
                     q += 1
             if gen_params:
                 from .waveforms import generate_gb
-                _, gscales = generate_gb(np.asarray(gen_params), gen_len, key[0], fmt="complex64",
+                fmt = "complex128" if self.precision == "mixed-c128" else "complex64"
+                _, gscales = generate_gb(np.asarray(gen_params), gen_len, key[0], fmt=fmt,
                                          t0s=gen_t0, out=arena, stream=stream, offsets=gen_off)
                 for q, sc in zip(gen_idx, gscales):
                     srcs[q].scale = float(sc)
@@ -325,8 +339,9 @@ class ResponseServer:
 
 # ------------------------------------------------------------------ synthetic traffic
 
-def gb_pool(num: int, n: int, dt: float, seed: int = 0) -> List[torch.Tensor]:
-    """`num` normalised complex64 galactic-binary waveforms of n samples, made on the device
+def gb_pool(num: int, n: int, dt: float, seed: int = 0,
+            dtype: torch.dtype = torch.complex64) -> List[torch.Tensor]:
+    """`num` normalised galactic-binary waveforms (max |h| = 1, `dtype`) of n samples, made on the device
     (f0 log-uniform in [1e-4, 1e-2] Hz, fdot log-uniform in [1e-19, 1e-15] Hz/s, cos iota,
     phi0, psi uniform)."""
     rng = np.random.default_rng(seed)
@@ -344,7 +359,7 @@ def gb_pool(num: int, n: int, dt: float, seed: int = 0) -> List[torch.Tensor]:
         c2, s2 = np.cos(2 * psi), np.sin(2 * psi)
         h = torch.complex(hp0 * c2 - hc0 * s2, hp0 * s2 + hc0 * c2)
         h = h / h.abs().max()
-        out.append(h.to(torch.complex64))
+        out.append(h.to(dtype))
     torch.cuda.synchronize()  # the server's streams read these
     return out
 
@@ -355,7 +370,7 @@ def poisson_requests(rng: np.random.Generator, rate: float, duration: float, poo
     """Arrival offsets (s) and requests: Poisson arrivals at `rate`/s over `duration` s, each a
     batch of 1..max_sources sources sharing one length, dt and TDI generation (probabilities
     in `gens`); sky uniform on the sphere, log10 amplitude uniform in [-23, -21]. With a `pool`
-    of complex64 waveforms each source takes a slice of one; without, each source carries
+    of waveforms each source takes a slice of one; without, each source carries
     galactic-binary parameters (f0 log-uniform in [1e-4, 1e-2] Hz, fdot log-uniform in
     [1e-19, 1e-15] Hz/s, cos i, phi0, psi uniform) and the server generates it."""
     times, reqs = [], []

@@ -8,7 +8,9 @@
   complex64-rounded input (isolates the kernel: only the fast form's ~1e-15 and the float32
   output rounding remain, rel-L2 <= 2e-7 per channel), and against the oracle on the FP64
   input (the storage error: rel-L2 <= 1e-6 on A and E at 5 mHz). The 32-bit fixed-point
-  input form ("mixed-q31") is pinned the same two ways.
+  input form ("mixed-q31") is pinned the same two ways. The complex128-input form
+  ("mixed-c128") has no storage error: against the oracle on the same FP64 input only the
+  float32 output rounding (~3e-8 rms relative) and the composite filters' delay tolerance remain.
 """
 import numpy as np
 import pytest
@@ -261,6 +263,55 @@ def test_q31_vs_fp64_oracle_at_5mhz(torch_gpu, orbits, oracle_orbits, oracle_mod
         assert r_c <= tol, (c, r_c)
 
 
+TOL_C128 = 1e-7  # mixed-c128 vs the oracle on the same FP64 input, every channel
+
+
+def test_c128_vs_fp64_oracle_at_5mhz(torch_gpu, orbits, oracle_orbits, oracle_mod):
+    """pyResponseTDI(precision="mixed-c128"): float32 outputs, no storage error, so the null
+    channel T is as accurate (relative) as A and E."""
+    torch = torch_gpu
+    from fastlisaresponse_amd import pyResponseTDI
+    n = 1 << 18
+    h = gb_waveform(n, 10.0, 1e-21, 5e-3, 1e-16, 0.8, 0.3, 0.5)
+    r = pyResponseTDI(0.1, n, orbits=orbits, tdi="2nd generation", tdi_chan="AET",
+                      precision="mixed-c128")
+    r.get_projections(h, 1.2, 0.3)
+    A, E, T = r.get_tdi_delays()
+    assert A.dtype == torch.float32
+    got = torch.stack([A, E, T]).cpu().numpy().astype(np.float64)
+    _, exp = oracle_mod.response_pipeline(oracle_orbits, h, 1.2, 0.3, 10.0, 25, "2nd generation",
+                                          x_base=orbits.x_base, nthreads=8)
+    for c in range(3):
+        r_c = rel_l2(got[c, 1000:-1000], exp[c, 1000:-1000])
+        assert r_c <= TOL_C128, (c, r_c)
+
+
+@pytest.mark.parametrize("tdi", ["1st generation", "2nd generation"])
+def test_c128_batch_vs_oracle(torch_gpu, orbits, oracle_orbits, oracle_mod, tdi):
+    """mixed-c128 batch of uneven sources with offsets in time, against the oracle on the same
+    complex128 input, XYZ and AET."""
+    torch = torch_gpu
+    from fastlisaresponse_amd.batch import BatchedResponse
+    rng = np.random.default_rng(53)
+    lengths = [40000, 9000, 25000]
+    hs, params = _sources(rng, lengths)
+    for chan in ("AET", "XYZ"):
+        br = BatchedResponse(0.1, tdi=tdi, orbits=orbits, tdi_chan=chan, precision="mixed-c128")
+        plan = _plan(torch, br, hs, params)
+        plan.run()
+        plan.check_errors()
+        assert plan.out_all.dtype == torch.float32
+        for s, n in enumerate(lengths):
+            lam, beta, t0 = params[s]
+            _, exp = oracle_mod.response_pipeline(oracle_orbits, hs[s], lam, beta, 10.0, 25, tdi,
+                                                  t0=t0, tdi_chan=chan, x_base=orbits.x_base,
+                                                  nthreads=8)
+            got = torch.stack(plan.channels(s)).cpu().numpy().astype(np.float64)
+            for c in range(3):
+                r = rel_l2(got[c, 1000:-1000], exp[c, 1000:-1000])
+                assert r <= TOL_C128, (chan, s, c, r)
+
+
 def test_fused_argument_checks(torch_gpu, orbits):
     torch = torch_gpu
     from fastlisaresponse_amd import pyResponseTDI
@@ -280,7 +331,7 @@ def test_fused_argument_checks(torch_gpu, orbits):
         plan.projections(0)
 
 
-@pytest.mark.parametrize("kind", ["mixed", "mixed-q31", "fused-fp64", "unfused"])
+@pytest.mark.parametrize("kind", ["mixed", "mixed-q31", "mixed-c128", "fused-fp64", "unfused"])
 def test_launch_shapes_give_identical_results(torch_gpu, orbits, kind):
     """Tile length and grid density change speed only: every shape the autotuner may pick gives
     the same bits."""
@@ -291,11 +342,12 @@ def test_launch_shapes_give_identical_results(torch_gpu, orbits, kind):
     lengths = [30000, 4000, 12000]
     hs, params = _sources(rng, lengths)
     kw = {"mixed": dict(precision="mixed"), "mixed-q31": dict(precision="mixed-q31"),
+          "mixed-c128": dict(precision="mixed-c128"),
           "unfused": dict(), "fused-fp64": dict(fused=True, exact=False)}[kind]
     br = BatchedResponse(0.1, tdi="2nd generation", orbits=orbits, tdi_chan="AET", **kw)
     scales = None
     h_all = None
-    if kind.startswith("mixed"):
+    if kind in ("mixed", "mixed-q31"):
         from fastlisaresponse_amd.batch import pack_q31
         pack = pack_q31 if kind == "mixed-q31" else pack_mixed
         h_all, scales = pack(torch.as_tensor(np.concatenate(hs), device="cuda"), lengths)
