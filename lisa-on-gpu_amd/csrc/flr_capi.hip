@@ -154,6 +154,7 @@ struct flr_batch {
     FusedSource *h_fused = nullptr;  // pinned staging of the source table and tile prefix
     int *h_fused_tiles = nullptr;
     hipEvent_t uploaded = nullptr;   // the last bind's upload
+    double *d_lti = nullptr;         // mixed fused: [capacity][2][kFir] composite TDI filters
 };
 
 namespace {
@@ -400,7 +401,8 @@ int make_fused_config(const flr_response *r, double dt, double fs, int flags, in
     tc.win_cap = tc.tile + halos;
     pc.win_cap = tc.win_cap + pc.halo_lo + pc.halo_hi;
     const bool generic = stock.generation == 0 || r->order != 25;
-    if (fused_lds_bytes(pc, tc, generic) > kMaxLds)
+    const bool lti = (flags & FLR_FLAG_MIXED) && !generic && stock.equal_arm;
+    if (fused_lds_bytes(pc, tc, generic, lti, passes) > kMaxLds)
         return fail(FLR_ERR_UNSUPPORTED, "fused delay windows exceed LDS (sampling too fine)");
     return FLR_OK;
 }
@@ -773,6 +775,7 @@ void flr_batch_destroy(flr_batch *b)
     }
     if (b->d_fused) (void)hipFree(b->d_fused);
     if (b->d_fused_tiles) (void)hipFree(b->d_fused_tiles);
+    if (b->d_lti) (void)hipFree(b->d_lti);
     if (b->h_fused) (void)hipHostFree(b->h_fused);
     if (b->h_fused_tiles) (void)hipHostFree(b->h_fused_tiles);
     delete b;
@@ -886,6 +889,8 @@ int flr_fused_reserve(flr_response *r, int capacity, double dt, double sampling_
     if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_fused, sizeof(FusedSource) * capacity);
     if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_fused_tiles, sizeof(int) * (capacity + 1));
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->uploaded, hipEventDisableTiming);
+    if (e == hipSuccess && (flags & FLR_FLAG_MIXED))
+        e = hipMalloc(&b->d_lti, sizeof(double) * 2 * kFir * capacity);
     if (e != hipSuccess) {
         flr_batch_destroy(b);
         return hip_fail(e, "fused batch reserve");
@@ -989,6 +994,7 @@ int flr_batch_run(flr_batch *b, void *stream)
         if ((st = make_fused_config(r, b->dt, b->fs, b->flags, b->tile_passes, pc, tc, stock)))
             return st;
         Batch<FusedSource> bf{b->d_fused, b->d_fused_tiles, b->num_sources, b->fused_tiles};
+        tc.lti_fir = b->d_lti;
         hipError_t e = launch_fused_batch(pc, tc, bf, r->orbits->view(), r->lagrange(), exact,
                                           (b->flags & FLR_FLAG_MIXED) != 0,
                                           (b->flags & FLR_FLAG_Q31) ? kInQ31

@@ -58,10 +58,123 @@ __device__ __forceinline__ void fused_store(const TdiConfig &tc, const FusedSour
     for (int c = 0; c < 3; ++c) store_out<OT>(src.out, c * n + i, o[c] * src.scale);
 }
 
+// The composite filters' reference for source src: its first TDI sample
+// (t0 + tdi_start * dt) and the arm length there.
+template <int NX, bool EXACT>
+__device__ __forceinline__ double lti_source_reference(const TdiConfig &tc, const FusedSource &src,
+                                                       const OrbitView &orb, const RowMap &rm,
+                                                       LtiRef<NX> &lref)
+{
+    const int T = src.tdi_start;
+    const double t_ref = src.t0 + (double)T * tc.dt;
+    const double L_ref = orbit_val(orbit_node<EXACT>(orb, t_ref), rm.row[0]);
+    lti_reference<NX>(t_ref, T, L_ref, src.t0, tc.fs, lref);
+    return L_ref;
+}
+
+// Unit in the last place of x > 0.
+__device__ __forceinline__ double ulp_of(double x) { return ldexp(1.0, ilogb(x) - 52); }
+
+// True when lti_matches holds for every sample of [first, last], so that the per-sample test
+// can be skipped (the outputs are then the same bits): (1) the filter's taps fit the window at
+// the tile's two ends, hence everywhere; (2) the arm length is L_ref, with zero slope, at every
+// orbit node the tile's times fall in, so each sample's chain delays t - m L differ from the
+// reference's only by rounding; (3) that rounding cannot move a position by kLtiTol: per
+// sample, t = t0 + i dt (1 ulp), m subtractions of L, the offset and the product with fs
+// each round by half an ulp of a magnitude below tmax (times fs), and dt fs may differ from 1
+// by r, which moves sample i by (i - T) r against the reference.
+template <int NX>
+__device__ __forceinline__ bool lti_tile_matches(const TdiConfig &tc, const FusedSource &src,
+                                                 const OrbitView &orb, const RowMap &rm,
+                                                 const LtiRef<NX> &ref, double L_ref, int first,
+                                                 int last)
+{
+    if (ref.kmin + tc.halo_lo < 0 || ref.kmin + kFir > tc.halo_hi + 1) return false;
+    const int row = rm.row[0];
+    const double *pa = orbit_node<false>(orb, src.t0 + (double)first * tc.dt).p;
+    const double *pb = orbit_node<false>(orb, src.t0 + (double)last * tc.dt).p;
+    for (const double *p = pa; p <= pb; p += kOrbitStride)
+        if (p[kOrbitVals + row] != 0.0 || p[row] != L_ref) return false;
+    constexpr int M = LtiRef<NX>::NG - 1;  // longest chain
+    const double tmax = 1.001 * (2.0 * fabs(src.t0) + (double)last * tc.dt + M * fabs(L_ref));
+    const double fs = tc.fs;
+    const double per_sample = (0.5 * M + 1.5) * ulp_of(tmax) * fs + ulp_of(tmax * fs);
+    const double drift = 1.01 * fabs(fma(tc.dt, fs, -1.0)) * (double)(last - src.tdi_start);
+    return 2.0 * per_sample + drift + 1e-13 <= kLtiTol;
+}
+
+// One workgroup per source: the two composite filters of the equal-arm stock TDI
+// ([2][kFir] doubles per source in tc.lti_fir; unused when the reference does not fit).
+template <int NX>
+__global__ __launch_bounds__(kBlock) void lti_filter_kernel(TdiConfig tc,
+                                                            Batch<FusedSource> batch,
+                                                            OrbitView orb, LagrangeView lg,
+                                                            RowMap rm)
+{
+    const int s = blockIdx.x;
+    if (s >= batch.num_sources || threadIdx.x >= 2 * kFir) return;
+    const FusedSource src = batch.sources[s];
+    LtiRef<NX> lref;
+    lti_source_reference<NX, false>(tc, src, orb, rm, lref);
+    tc.lti_fir[(size_t)s * (2 * kFir) + threadIdx.x] =
+        lref.ok ? lti_filter_entry<NX>(lref, lg, threadIdx.x) : 0.0;
+}
+
+// Composite-filter TDI of one tile on the matrix cores. Outputs are taken in blocks of 16
+// aligned to the source's first TDI sample (block q = samples T + 16q .. T + 16q + 15, so a
+// sample's arithmetic is the same whatever tile holds it); per output o and block,
+//   out[i_b + r] = sum_kind sum_{m=0}^{55} F_kind[m - r] S_{o,kind}[i_b + kmin + m],
+// which is D = A B with A[r][m] = F[m - r] (Toeplitz, the padded filters) and
+// B[m][block] = the series: one v_mfma_f64_16x16x4f64 per 4 taps of 16 blocks x 16 outputs
+// (28 per group of 16 blocks of one output). Operand layout: lane l holds A[l % 16][l / 16]
+// and B[l / 16][l % 16]; the result's element v is D[4v + l / 16][l % 16]. Tap columns are
+// clamped into the window, so edge blocks read finite samples wherever F is zero; their rows
+// outside the tile, and samples left to the per-sample path (lti_ok = 0 unless all_ok), are
+// not stored.
+template <typename OT>
+__device__ __forceinline__ void lti_mfma(const FusedSource &src, int first, int last, int i0,
+                                         int yw_lo, int yw_len, int kmin, const double *ywin,
+                                         int pitch, const double *fpad,
+                                         const unsigned char *lti_ok, bool all_ok)
+{
+    typedef double d4 __attribute__((ext_vector_type(4)));
+    const int T = src.tdi_start;
+    const int q0 = (first - T) >> 4, q1 = (last - T) >> 4;  // first >= T
+    const int ngc = (q1 - q0 + 16) >> 4;                     // groups of 16 blocks per output
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nb = lane & 15, kq = lane >> 4;
+    const size_t n = (size_t)src.num_pts;
+    for (int g = wave; g < 3 * ngc; g += kBlock / 64) {
+        const int o = g / ngc;
+        const int qb = q0 + 16 * (g - o * ngc) + nb;
+        const int wb = T + 16 * min(qb, q1) + kmin - yw_lo;  // window column of tap m = 0
+        d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kind = 0; kind < 2; ++kind) {
+            const double *row = ywin + (2 * o + kind) * pitch;
+            const double *f = fpad + kind * kFpad + kFpadLo + kq - nb;  // f[4j] = F[4j + kq - nb]
+#pragma unroll
+            for (int j = 0; j < 14; ++j) {
+                const int w = min(max(wb + 4 * j + kq, 0), yw_len - 1);
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(f[4 * j], row[lti_col(w)], acc, 0, 0, 0);
+                __asm__ volatile("" ::: "memory");  // one operand pair in flight: no register growth
+            }
+        }
+        if (qb > q1) continue;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = T + 16 * qb + 4 * v + kq;
+            if (i >= first && i <= last && (all_ok || lti_ok[i - i0]))
+                store_out<OT>(src.out, o * n + i, acc[v] * src.scale);
+        }
+    }
+}
+
 template <bool EXACT, int MODE, int NX, typename HT, typename OT, int PASSES>
 __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig &tc,
-                                           const FusedSource &src, int tile, const OrbitView &orb,
-                                           const LagrangeView &lg, const RowMap &rm, double *smem)
+                                           const FusedSource &src, int s, int tile,
+                                           const OrbitView &orb, const LagrangeView &lg,
+                                           const RowMap &rm, double *smem)
 {
     constexpr int H = 13;
     constexpr int kPasses = PASSES;
@@ -94,10 +207,18 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
         }
         return;
     }
+    // Mixed precision, equal arms: the stock TDI is two composite time-invariant filters per
+    // output (built once per source by lti_filter_kernel, referenced at the source's first TDI
+    // sample so that every tile applies the same filter) applied on the matrix cores; the
+    // window rows are then padded (lti_col) for the matrix operands' loads.
+    constexpr bool kLti = MODE == kModeLinear && std::is_same<OT, float>::value;
+    const int pitch = kLti ? lti_pitch(cap) : cap;
     const int hw_lo = max(0, yw_lo - pc.halo_lo);
     const int hw_len = min(n, yw_lo + yw_len + pc.halo_hi) - hw_lo;  // <= pc.win_cap (host-sized)
     double2 *hwin = reinterpret_cast<double2 *>(smem);
-    double *ywin = smem + 2 * pc.win_cap;  // [6][cap]
+    double *ywin = smem + 2 * pc.win_cap;  // [6][pitch]
+    double *fpad = ywin + kNLinks * pitch;  // kLti: [2][kFpad]
+    unsigned char *lti_ok = reinterpret_cast<unsigned char *>(fpad + 2 * kFpad);  // [PASSES * kBlock]
     for (int jj = tid; jj < hw_len; jj += kBlock) hwin[jj] = load_h<HT>(src.h, hw_lo + jj);
     __syncthreads();
 
@@ -106,49 +227,54 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
     bool bad = false;
     for (int w = tid; w < yw_len; w += kBlock) {
         const int j = yw_lo + w;
-        double *yc = ywin + w;
+        int col = kLti ? lti_col(w) : w;
+        pin(col);  // an opaque column: the padded index is not re-derived inside the projection
+        double *yc = ywin + col;
         if (j < src.proj_start || j >= n - src.proj_start) {
 #pragma unroll
-            for (int r = 0; r < kNLinks; ++r) yc[r * cap] = 0.0;
+            for (int r = 0; r < kNLinks; ++r) yc[r * pitch] = 0.0;
         } else {
             const double t = src.t0 + (double)j * pc.dt;
             bad |= project_links<EXACT, H, MODE == kModeGroups || MODE == kModeLinear>(pc, src.k, src.u, src.v, t, t0_offset, orb, lg, hwin,
                                            hw_lo, hw_len, y_deps,
-                                           [&](int row, double val) { yc[row * cap] = val; });
+                                           [&](int row, double val) { yc[row * pitch] = val; });
         }
-        if constexpr (MODE == kModeLinear) linear_series(yc, cap, rm, tc.aet, tc.s2, tc.s6, tc.s3);
+        if constexpr (MODE == kModeLinear) linear_series(yc, pitch, rm, tc.aet, tc.s2, tc.s6, tc.s3);
     }
     __syncthreads();
 
-    // Mixed precision, equal arms: composite time-invariant filters, referenced at the
-    // source's first TDI sample so that every tile of a source (whatever the tile length or
-    // the batch) applies the same filter.
-    constexpr bool kLti = MODE == kModeLinear && std::is_same<OT, float>::value;
-    double *fir = ywin + kNLinks * cap;  // [2][kFir]
-    LtiRef<NX> lref;
-    bool lti = false;
+    LtiRef<NX> lref;  // set up after the projection: not live across it
+    bool lti = false, tile_ok = false;
     if constexpr (kLti) {
-        const double t_ref = src.t0 + (double)T * tc.dt;
-        const double L_ref = orbit_val(orbit_node<EXACT>(orb, t_ref), rm.row[0]);
-        lti_reference<NX>(t_ref, T, L_ref, t0_offset, tc.fs, lref);
+        const double L_ref = lti_source_reference<NX, EXACT>(tc, src, orb, rm, lref);
         lti = lref.ok;
-        if (lti && tid < 2 * kFir) fir[tid] = lti_filter_entry<NX>(lref, lg, tid);
-        __syncthreads();
+        tile_ok = lti && lti_tile_matches<NX>(tc, src, orb, rm, lref, L_ref, first, last);
+        const double *fir = tc.lti_fir + (size_t)s * (2 * kFir);
+        if (lti) {  // read by lti_mfma after the barrier that follows the per-sample pass
+            for (int x = tid; x < 2 * kFpad; x += kBlock) {
+                const int kind = x / kFpad, k = x - kind * kFpad - kFpadLo;
+                fpad[x] = (k >= 0 && k < kFir) ? fir[kind * kFir + k] : 0.0;
+            }
+        }
     }
 
 #pragma unroll 1
-    for (int r = 0; r < kPasses; ++r) {
+    for (int r = 0; r < (tile_ok ? 0 : kPasses); ++r) {  // tile_ok: every sample goes below
         const int i = i0 + r * kBlock + tid;
         if (i < first || i > last) continue;
         const double t = src.t0 + (double)i * tc.dt;
         double acc[3] = {0.0, 0.0, 0.0};
         if constexpr (kLti) {
+            // samples whose delays match the reference are left to the matrix-core pass below;
+            // the others (if any) take the per-sample stencils here
             const double L = orbit_val(orbit_node<EXACT>(orb, t), rm.row[0]);
-            if (lti && lti_matches<NX>(t, i, L, t0_offset, tc.fs, lref, yw_lo, yw_len)) {
-                lti_apply(ywin, cap, i + lref.kmin - yw_lo, fir, acc);
-                fused_store<OT>(tc, src, i, acc, false);
-                continue;
-            }
+            const bool ok = lti && lti_matches<NX>(t, i, L, t0_offset, tc.fs, lref, yw_lo, yw_len);
+            lti_ok[i - i0] = ok;
+            if (ok) continue;
+            bad |= !tdi_linear_padded<NX>(t, L, t0_offset, tc.fs, ywin, pitch, yw_lo, yw_len, lg,
+                                          acc);
+            fused_store<OT>(tc, src, i, acc, false);
+            continue;
         }
         if constexpr (MODE == kModeGeneric) {
             double *Ls = ywin + kNLinks * cap;  // [6][kBlock], one column per lane
@@ -181,6 +307,13 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
         }
         fused_store<OT>(tc, src, i, acc, MODE != kModeLinear && tc.aet);
     }
+    if constexpr (kLti) {
+        if (lti) {
+            __syncthreads();  // lti_ok written
+            lti_mfma<OT>(src, first, last, i0, yw_lo, yw_len, lref.kmin, ywin, pitch, fpad, lti_ok,
+                         tile_ok);
+        }
+    }
     if (bad) atomicOr(tc.err, kErrWindow);
 }
 
@@ -191,7 +324,18 @@ __global__ __launch_bounds__(kBlock) void fused_batch_kernel(ProjConfig pc, TdiC
                                                              RowMap rm)
 {
     extern __shared__ double smem[];
-    FLR_BATCH_LOOP(batch, (fused_tile<EXACT, MODE, NX, HT, OT, PASSES>(pc, tc, src, tile, orb, lg, rm, smem)));
+    FLR_BATCH_LOOP(batch, (fused_tile<EXACT, MODE, NX, HT, OT, PASSES>(pc, tc, src, s_, tile, orb, lg, rm, smem)));
+}
+
+// The mixed equal-arm form (composite filters on the matrix cores), held to 4 waves per SIMD:
+// its VGPRs and the matrix accumulator share one 512-entry register file per lane.
+template <int NX, typename HT, int PASSES>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
+void fused_lti_kernel(ProjConfig pc, TdiConfig tc, Batch<FusedSource> batch, OrbitView orb,
+                      LagrangeView lg, RowMap rm)
+{
+    extern __shared__ double smem[];
+    FLR_BATCH_LOOP(batch, (fused_tile<false, kModeLinear, NX, HT, float, PASSES>(pc, tc, src, s_, tile, orb, lg, rm, smem)));
 }
 
 // Launch one fused variant over a batch: the TDI mode follows the table (stock + equal arms:
@@ -202,12 +346,26 @@ hipError_t launch_fused_variant(const ProjConfig &pc, const TdiConfig &tc, const
                                 int grid_per_cu, hipStream_t stream)
 {
     const int nb = batch_grid(b.total_tiles, grid_per_cu);
-    const size_t shm = fused_lds_bytes(pc, tc, !use_stock(st, lg));
+    const bool lti = std::is_same<OT, float>::value && !EXACT && use_equal_arm(st, lg);
+    const size_t shm = fused_lds_bytes(pc, tc, !use_stock(st, lg), lti, PASSES);
     const RowMap &rm = st.rows;
     constexpr int kEq = EXACT ? kModeGroups : kModeLinear;
 #define FLR_FUSED(M, NX) \
     launch(fused_batch_kernel<EXACT, M, NX, HT, OT, PASSES>, nb, shm, stream, pc, tc, b, orb, lg, rm)
-    if (use_equal_arm(st, lg)) return st.generation == 1 ? FLR_FUSED(kEq, 8) : FLR_FUSED(kEq, 16);
+    if (use_equal_arm(st, lg)) {
+        if constexpr (kEq == kModeLinear && std::is_same<OT, float>::value) {
+            if (!tc.lti_fir) return hipErrorInvalidValue;
+            const bool g1 = st.generation == 1;
+            const hipError_t e =  // the composite filters first, then the tiles
+                g1 ? launch(lti_filter_kernel<8>, b.num_sources, 0, stream, tc, b, orb, lg, rm)
+                   : launch(lti_filter_kernel<16>, b.num_sources, 0, stream, tc, b, orb, lg, rm);
+            if (e != hipSuccess) return e;
+            return g1 ? launch(fused_lti_kernel<8, HT, PASSES>, nb, shm, stream, pc, tc, b, orb, lg, rm)
+                      : launch(fused_lti_kernel<16, HT, PASSES>, nb, shm, stream, pc, tc, b, orb, lg, rm);
+        } else {
+            return st.generation == 1 ? FLR_FUSED(kEq, 8) : FLR_FUSED(kEq, 16);
+        }
+    }
     if (use_stock(st, lg))
         return st.generation == 1 ? FLR_FUSED(kModeStock, 8) : FLR_FUSED(kModeStock, 16);
     return FLR_FUSED(kModeGeneric, 0);

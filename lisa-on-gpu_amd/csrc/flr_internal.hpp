@@ -87,6 +87,8 @@ struct TdiConfig {
     double s2, s6, s3;      // sqrt(2), sqrt(6), sqrt(3)
     int *err;
     int tile;               // fused launches: output samples per tile
+    double *lti_fir = nullptr;  // mixed fused launches: per source, the two composite TDI filters
+                                // ([2][kFir] doubles, written by lti_filter_kernel before the launch)
 };
 
 struct TdiSource {
@@ -192,7 +194,10 @@ hipError_t launch_fused_c128(const ProjConfig &pc, const TdiConfig &tc,
                              const Batch<FusedSource> &batch, const OrbitView &orb,
                              const LagrangeView &lg, int tile_passes, int grid_per_cu,
                              const StockTdi &stock, hipStream_t stream);
-size_t fused_lds_bytes(const ProjConfig &pc, const TdiConfig &tc, bool generic);
+// Dynamic LDS of a fused launch: h window, six window rows (padded, plus the matrix-core
+// filters and per-sample flags, for the mixed equal-arm form), generic TDI scratch.
+size_t fused_lds_bytes(const ProjConfig &pc, const TdiConfig &tc, bool generic, bool lti,
+                       int passes);
 // Galactic-binary generator (flr_waveform.hip).
 hipError_t launch_gb(const flr_gb_source *sources, int num_sources, int64_t total, double dt,
                      int format, void *out, hipStream_t stream);

@@ -390,11 +390,14 @@ hipError_t launch_clear(int *word, hipStream_t stream)
     return hipGetLastError();
 }
 
-size_t fused_lds_bytes(const ProjConfig &pc, const TdiConfig &tc, bool generic)
+size_t fused_lds_bytes(const ProjConfig &pc, const TdiConfig &tc, bool generic, bool lti,
+                       int passes)
 {
-    return (size_t)pc.win_cap * sizeof(double2) +
-           ((size_t)kNLinks * tc.win_cap + (generic ? (size_t)kNLinks * kBlock : (size_t)2 * kFir)) *
-               sizeof(double);
+    const size_t doubles =
+        lti ? (size_t)kNLinks * lti_pitch(tc.win_cap) + 2 * kFpad
+            : (size_t)kNLinks * tc.win_cap + (generic ? (size_t)kNLinks * kBlock : 0);
+    return (size_t)pc.win_cap * sizeof(double2) + doubles * sizeof(double) +
+           (lti ? (size_t)passes * kBlock : 0);
 }
 
 hipError_t launch_fused_batch(const ProjConfig &pc, const TdiConfig &tc,

@@ -315,11 +315,13 @@ __device__ __forceinline__ void linear_series(double *yc, int cap, const RowMap 
 // Equal-arm stock TDI from the combined series (fast stencils): per chain length m one delay
 // t - m*L, one set of stencil weights and one stencil per output -- 3 stencils per group
 // instead of 6 units. Equal to the unit-by-unit sum up to rounding. Returns false if a stencil
-// fell outside the window (the outputs are NaN).
-template <int NX>
-__device__ __forceinline__ bool tdi_linear(double t, double L, double t0_offset, double fs,
-                                           const double *ywin, int cap, int win_lo, int win_len,
-                                           const LagrangeView &lg, double (&acc)[3])
+// fell outside the window (the outputs are NaN). The taps are read through
+// cols(o, kind, c0) -> a pointer p with p[k] = window column c0 + k of output o's series of
+// that kind (k = 0 .. 2H-1).
+template <int NX, typename Cols>
+__device__ __forceinline__ bool tdi_linear_cols(double t, double L, double t0_offset, double fs,
+                                                int win_lo, int win_len, const LagrangeView &lg,
+                                                Cols cols, double (&acc)[3])
 {
     constexpr int H = 13;
     constexpr int NG = stock::max_len(NX) + 1;
@@ -334,13 +336,9 @@ __device__ __forceinline__ bool tdi_linear(double t, double L, double t0_offset,
         ok = ok && !gbad;
         if (gbad) d = win_lo + H - 1;  // in-window stand-in; the result is discarded
         constexpr int kind = stock::group_kind(m);
-        const double *xr[3];
-#pragma unroll
-        for (int o = 0; o < 3; ++o) {
-            int off = (2 * o + kind) * cap + d - win_lo - (H - 1);
-            pin(off);
-            xr[o] = ywin + off;  // lowest tap; every tap at a non-negative immediate offset
-        }
+        const int c0 = d - win_lo - (H - 1);  // lowest tap
+        using Taps = decltype(cols(0, kind, c0));
+        const Taps xr[3] = {cols(0, kind, c0), cols(1, kind, c0), cols(2, kind, c0)};
         double S[3] = {0.0, 0.0, 0.0};
         double P = 1.0;
 #pragma unroll
@@ -351,7 +349,8 @@ __device__ __forceinline__ bool tdi_linear(double t, double L, double t0_offset,
             const double EP = lg.E[j - 1] * P;
 #pragma unroll
             for (int o = 0; o < 3; ++o) {
-                const double num = __builtin_fma(xr[o][H + j], qd, xr[o][H - 1 - j] * qu);
+                const Taps &x = xr[o];
+                const double num = __builtin_fma(x[H + j], qd, x[H - 1 - j] * qu);
                 S[o] = __builtin_fma(S[o], Q, num * EP);
                 pin(S[o]);
             }
@@ -366,13 +365,56 @@ __device__ __forceinline__ bool tdi_linear(double t, double L, double t0_offset,
         constexpr double sgn = (double)stock::group_sign(m);
 #pragma unroll
         for (int o = 0; o < 3; ++o) {
-            const double *p = xr[o] + (H - 1);  // p[0] = x[d], p[1] = x[d+1]
-            const double val = A * __builtin_fma(D, S[o] * invP, __builtin_fma(B, p[1], e * p[0]));
+            const Taps &x = xr[o];  // x[H - 1] = sample d, x[H] = d + 1
+            const double val = A * __builtin_fma(D, S[o] * invP, __builtin_fma(B, x[H], e * x[H - 1]));
             acc[o] += gbad ? __builtin_nan("") : sgn * val;
             pin(acc[o]);
         }
     });
     return ok;
+}
+
+// Contiguous rows of pitch cap: every tap at a non-negative immediate offset from one pinned
+// base address per output.
+template <int NX>
+__device__ __forceinline__ bool tdi_linear(double t, double L, double t0_offset, double fs,
+                                           const double *ywin, int cap, int win_lo, int win_len,
+                                           const LagrangeView &lg, double (&acc)[3])
+{
+    return tdi_linear_cols<NX>(t, L, t0_offset, fs, win_lo, win_len, lg,
+                               [&](int o, int kind, int c0) {
+                                   int off = (2 * o + kind) * cap + c0;
+                                   pin(off);
+                                   return ywin + off;
+                               },
+                               acc);
+}
+
+// Column w of a padded window row lives at w + w / 16 (one spare double per 16 samples), so that
+// 16 lanes reading columns 16 apart hit 16 different LDS bank pairs; rows have pitch
+// lti_pitch(cap).
+__host__ __device__ constexpr int lti_col(int w) { return w + (w >> 4); }
+__host__ __device__ constexpr int lti_pitch(int cap) { return cap + (cap >> 4) + 1; }
+
+// Padded window column c0 + k of a row, read through operator[].
+struct PaddedTaps {
+    const double *row;
+    int c0;
+    __device__ __forceinline__ double operator[](int k) const { return row[lti_col(c0 + k)]; }
+};
+
+// tdi_linear on padded rows (the per-sample fallback of the matrix-core path).
+template <int NX>
+__device__ __forceinline__ bool tdi_linear_padded(double t, double L, double t0_offset, double fs,
+                                                  const double *ywin, int pitch, int win_lo,
+                                                  int win_len, const LagrangeView &lg,
+                                                  double (&acc)[3])
+{
+    return tdi_linear_cols<NX>(t, L, t0_offset, fs, win_lo, win_len, lg,
+                               [&](int o, int kind, int c0) {
+                                   return PaddedTaps{ywin + (2 * o + kind) * pitch, c0};
+                               },
+                               acc);
 }
 
 // ------------------------------------------------------------------ time-invariant form
@@ -386,6 +428,11 @@ __device__ __forceinline__ bool tdi_linear(double t, double L, double t0_offset,
 // kLtiTol * |dx/dsample| per stencil -- used by the mixed-precision kernels only.
 constexpr int kFir = 40;
 constexpr double kLtiTol = 1e-8;
+// One kind's composite filter as the matrix-core path reads it: taps -15 .. 55 (zero outside
+// 0 .. kFir-1), so that a 16-output block's Toeplitz operand F[q - r] (q in 0..55, r in 0..15)
+// is one indexed load.
+constexpr int kFpadLo = 15;
+constexpr int kFpad = 72;
 
 template <int NX>
 struct LtiRef {
@@ -458,29 +505,6 @@ __device__ __forceinline__ bool lti_matches(double t, int i, double L, double t0
         ok = ok && fabs(delta - ref.delta[m]) <= kLtiTol;
     });
     return ok;
-}
-
-// acc[o] = sum_k fir[k] s_{o,0}[k] + fir[kFir + k] s_{o,1}[k], s_{o,kind} = row 2*o + kind of
-// the combined-series window starting at column `base`.
-__device__ __forceinline__ void lti_apply(const double *ywin, int cap, int base, const double *fir,
-                                          double (&acc)[3])
-{
-    int off[6];
-#pragma unroll
-    for (int s = 0; s < 6; ++s) {
-        off[s] = s * cap + base;
-        pin(off[s]);
-    }
-#pragma unroll
-    for (int k = 0; k < kFir; ++k) {
-        const double wa = fir[k];
-        const double wb = fir[kFir + k];
-#pragma unroll
-        for (int o = 0; o < 3; ++o) {
-            acc[o] = __builtin_fma(wa, ywin[off[2 * o] + k], acc[o]);
-            acc[o] = __builtin_fma(wb, ywin[off[2 * o + 1] + k], acc[o]);
-        }
-    }
 }
 
 }  // namespace flr

@@ -286,6 +286,46 @@ def test_c128_vs_fp64_oracle_at_5mhz(torch_gpu, orbits, oracle_orbits, oracle_mo
         assert r_c <= TOL_C128, (c, r_c)
 
 
+def test_mixed_far_times_take_the_per_sample_path(torch_gpu, oracle_mod):
+    """At t ~ 1e9 s the rounding of the delays can exceed the composite filters' tolerance, so
+    tiles are not cleared as a whole and samples are tested one by one; those that fail take
+    the per-sample stencils on the padded window. Against the oracle, and across launch shapes
+    (bitwise), for the complex128 and complex64 inputs."""
+    torch = torch_gpu
+    from fastlisaresponse_amd.autotune import apply, candidates
+    from fastlisaresponse_amd.batch import BatchedResponse, pack_mixed
+    from fastlisaresponse_amd.orbits import EqualArmlengthOrbits
+    orb = EqualArmlengthOrbits(t_max=1.0e9 + 2.0e6)
+    orb.configure(linear_interp_setup=True)
+    oo = oracle_mod.OracleOrbits(*orb.pycppdetector_args)
+    n, t0 = 60000, 1.0e9
+    h = gb_waveform(n, 10.0, 1e-21, 4e-3, 1e-17, 0.7, 0.2, 0.4, t0=t0)
+    _, exp = oracle_mod.response_pipeline(oo, h, 0.9, -0.4, 10.0, 25, "2nd generation", t0=t0,
+                                          x_base=orb.x_base, nthreads=8)
+    hd = torch.as_tensor(h, device="cuda")
+    for precision in ("mixed-c128", "mixed"):
+        br = BatchedResponse(0.1, tdi="2nd generation", orbits=orb, precision=precision)
+        if precision == "mixed":
+            hin, sc = pack_mixed(hd, [n])
+        else:
+            hin, sc = hd, None
+        plan = br.plan(hin, [n], [0.9], [-0.4], [t0], scales=sc)
+        ref = None
+        for shape in candidates(plan):
+            apply(plan, *shape)
+            plan.out_all.fill_(float("nan"))
+            plan.run()
+            plan.check_errors()
+            if ref is None:
+                ref = plan.out_all.clone()
+            else:
+                assert torch.equal(plan.out_all, ref), (precision, shape)
+        got = ref.view(3, n).double().cpu().numpy()
+        tol = TOL_C128 if precision == "mixed-c128" else 1e-6
+        for c in range(2):
+            assert rel_l2(got[c, 1000:-1000], exp[c, 1000:-1000]) <= tol, (precision, c)
+
+
 @pytest.mark.parametrize("tdi", ["1st generation", "2nd generation"])
 def test_c128_batch_vs_oracle(torch_gpu, orbits, oracle_orbits, oracle_mod, tdi):
     """mixed-c128 batch of uneven sources with offsets in time, against the oracle on the same
