@@ -93,6 +93,7 @@ struct flr_orbits {
 
 struct flr_response {
     flr_orbits *orbits = nullptr;
+    uint64_t version = 0;  // bumped when orbits, Lagrange or TDI tables change
     int device = 0;
     int *d_err = nullptr;
     bool exact_default = true;
@@ -155,6 +156,7 @@ struct flr_batch {
     int *h_fused_tiles = nullptr;
     hipEvent_t uploaded = nullptr;   // the last bind's upload
     double *d_lti = nullptr;         // mixed fused: [capacity][2][kFir] composite TDI filters
+    uint64_t lti_version = ~0ull;    // response version the filters were built for (none yet)
 };
 
 namespace {
@@ -197,6 +199,7 @@ int set_lagrange_impl(flr_response *r, int order, const double *A_in, double dep
     r->A_host = std::move(A);
     r->E_host = std::move(E);
     r->lag_set = true;
+    ++r->version;
     return FLR_OK;
 }
 
@@ -318,6 +321,7 @@ int set_tdi_impl(flr_response *r, const int32_t *unit_starts, const int32_t *uni
     r->sig_comb = std::move(comb);
     r->sig_signs = std::move(signs);
     r->tdi_set = true;
+    ++r->version;
     return FLR_OK;
 }
 
@@ -630,6 +634,7 @@ int flr_response_create(flr_orbits *orbits, flr_response **out)
     auto r = new flr_response();
     r->orbits = orbits;
     r->device = orbits->device;
+    ++r->version;
     hipError_t e = hipMalloc(&r->d_err, sizeof(int));
     if (e == hipSuccess) {
         const int zero = 0;
@@ -899,6 +904,32 @@ int flr_fused_reserve(flr_response *r, int capacity, double dt, double sampling_
     return FLR_OK;
 }
 
+namespace {
+
+// Build the bound sources' composite TDI filters (mixed fused batches with equal-arm stock
+// tables) unless they are current: they are set on bind and rebuilt when the response's
+// orbits, Lagrange or TDI tables have changed since. Enqueued on `stream`.
+int lti_refresh(flr_batch *b, hipStream_t stream)
+{
+    flr_response *r = b->r;
+    if (!b->fused || !(b->flags & FLR_FLAG_MIXED) || b->num_sources < 1 ||
+        b->lti_version == r->version)
+        return FLR_OK;
+    ProjConfig pc;
+    TdiConfig tc;
+    StockTdi stock;
+    int st = make_fused_config(r, b->dt, b->fs, b->flags, b->tile_passes, pc, tc, stock);
+    if (st) return st;
+    tc.lti_fir = b->d_lti;
+    Batch<FusedSource> bf{b->d_fused, b->d_fused_tiles, b->num_sources, b->fused_tiles};
+    hipError_t e = launch_lti_filters(tc, bf, r->orbits->view(), r->lagrange(), stock, stream);
+    if (e != hipSuccess) return hip_fail(e, "composite filter launch");
+    b->lti_version = r->version;
+    return FLR_OK;
+}
+
+}  // namespace
+
 int flr_fused_bind(flr_batch *b, const flr_source *sources, int num_sources, const void *h_all,
                    void *out_all, void *stream)
 {
@@ -949,11 +980,14 @@ int flr_fused_bind(flr_batch *b, const flr_source *sources, int num_sources, con
                            hipMemcpyHostToDevice, st));
     FLR_HIP(hipMemcpyAsync(b->d_fused_tiles, b->h_fused_tiles, sizeof(int) * (num_sources + 1),
                            hipMemcpyHostToDevice, st));
-    FLR_HIP(hipEventRecord(b->uploaded, st));
     b->num_sources = num_sources;
     b->samples = samples;
     b->fused_tiles = ft[num_sources];
-    return FLR_OK;
+    b->lti_version = ~0ull;  // new sources: their filters are built now, after the upload
+    const int fst = lti_refresh(b, st);
+    // recorded after the filters too: flr_fused_create waits on it before returning
+    FLR_HIP(hipEventRecord(b->uploaded, st));
+    return fst;
 }
 
 int flr_fused_create(flr_response *r, const flr_source *sources, int num_sources,
@@ -991,6 +1025,7 @@ int flr_batch_run(flr_batch *b, void *stream)
     StockTdi stock;
     int st;
     if (b->fused) {
+        if ((st = lti_refresh(b, s))) return st;
         if ((st = make_fused_config(r, b->dt, b->fs, b->flags, b->tile_passes, pc, tc, stock)))
             return st;
         Batch<FusedSource> bf{b->d_fused, b->d_fused_tiles, b->num_sources, b->fused_tiles};

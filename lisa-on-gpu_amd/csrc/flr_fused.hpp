@@ -354,14 +354,11 @@ hipError_t launch_fused_variant(const ProjConfig &pc, const TdiConfig &tc, const
     launch(fused_batch_kernel<EXACT, M, NX, HT, OT, PASSES>, nb, shm, stream, pc, tc, b, orb, lg, rm)
     if (use_equal_arm(st, lg)) {
         if constexpr (kEq == kModeLinear && std::is_same<OT, float>::value) {
+            // tc.lti_fir holds the sources' composite filters (launch_lti_filters)
             if (!tc.lti_fir) return hipErrorInvalidValue;
-            const bool g1 = st.generation == 1;
-            const hipError_t e =  // the composite filters first, then the tiles
-                g1 ? launch(lti_filter_kernel<8>, b.num_sources, 0, stream, tc, b, orb, lg, rm)
-                   : launch(lti_filter_kernel<16>, b.num_sources, 0, stream, tc, b, orb, lg, rm);
-            if (e != hipSuccess) return e;
-            return g1 ? launch(fused_lti_kernel<8, HT, PASSES>, nb, shm, stream, pc, tc, b, orb, lg, rm)
-                      : launch(fused_lti_kernel<16, HT, PASSES>, nb, shm, stream, pc, tc, b, orb, lg, rm);
+            return st.generation == 1
+                       ? launch(fused_lti_kernel<8, HT, PASSES>, nb, shm, stream, pc, tc, b, orb, lg, rm)
+                       : launch(fused_lti_kernel<16, HT, PASSES>, nb, shm, stream, pc, tc, b, orb, lg, rm);
         } else {
             return st.generation == 1 ? FLR_FUSED(kEq, 8) : FLR_FUSED(kEq, 16);
         }

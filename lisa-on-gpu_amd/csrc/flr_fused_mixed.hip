@@ -25,4 +25,20 @@ hipError_t launch_fused_mixed(const ProjConfig &pc, const TdiConfig &tc,
     }
 }
 
+bool lti_filters_needed(const StockTdi &st, const LagrangeView &lg)
+{
+    return use_equal_arm(st, lg);
+}
+
+hipError_t launch_lti_filters(const TdiConfig &tc, const Batch<FusedSource> &b,
+                              const OrbitView &orb, const LagrangeView &lg, const StockTdi &st,
+                              hipStream_t stream)
+{
+    if (!lti_filters_needed(st, lg)) return hipSuccess;
+    if (!tc.lti_fir) return hipErrorInvalidValue;
+    return st.generation == 1
+               ? launch(lti_filter_kernel<8>, b.num_sources, 0, stream, tc, b, orb, lg, st.rows)
+               : launch(lti_filter_kernel<16>, b.num_sources, 0, stream, tc, b, orb, lg, st.rows);
+}
+
 }  // namespace flr

@@ -190,6 +190,12 @@ hipError_t launch_fused_q31(const ProjConfig &pc, const TdiConfig &tc,
                             const Batch<FusedSource> &batch, const OrbitView &orb,
                             const LagrangeView &lg, int tile_passes, int grid_per_cu,
                             const StockTdi &stock, hipStream_t stream);
+// The composite TDI filters of a mixed fused batch's sources (equal-arm stock tables only),
+// into tc.lti_fir; they depend on the sources, orbits, Lagrange and TDI tables, not on h.
+bool lti_filters_needed(const StockTdi &stock, const LagrangeView &lg);
+hipError_t launch_lti_filters(const TdiConfig &tc, const Batch<FusedSource> &batch,
+                              const OrbitView &orb, const LagrangeView &lg,
+                              const StockTdi &stock, hipStream_t stream);
 hipError_t launch_fused_c128(const ProjConfig &pc, const TdiConfig &tc,
                              const Batch<FusedSource> &batch, const OrbitView &orb,
                              const LagrangeView &lg, int tile_passes, int grid_per_cu,
