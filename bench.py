@@ -388,7 +388,8 @@ def main() -> int:
         line["roofline"] = {
             "bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
-            "kernel": f"fused_batch_kernel ({args.precision})", "kernel_ms": ms,
+            "kernel": ("fused_lti_kernel" if args.orbits == "equal" else "fused_batch_kernel")
+                      + f" ({args.precision})", "kernel_ms": ms,
             "algorithmic_flop_per_sample": flop, "algorithmic_bytes_per_sample": nbytes,
             "hbm_achieved_GBs": nbytes * N_SAMPLES / (ms * 1e-3) / 1e9,
             "hbm_frac": nbytes * N_SAMPLES / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
