@@ -422,6 +422,23 @@ def main() -> int:
         elif "value" in line:
             line["mixed_vs_fp64"] = line["value"] / fp64["value"]
 
+    if args.only is None and args.precision == "mixed-c128":
+        # the same form through the drop-in API (pyResponseTDI, one reserved batch rebound per
+        # call): what a reference user switching precision= gets, host work included
+        rd = pyResponseTDI(1 / DT, N_SAMPLES, order=ORDER, tdi=args.tdi, orbits=orbits,
+                           tdi_chan="AET", precision="mixed-c128")
+
+        def dropin_step():
+            rd.get_projections(h, lam, beta, t0=0.0, t_buffer=T_BUFFER)
+            rd.get_tdi_delays()
+
+        wd, _ = timed(dropin_step, args.steps, args.warmup)
+        rd.check_errors()
+        line["mixed_c128_dropin"] = {"value": N_SAMPLES * args.steps / wd,
+                                     "ms_per_step": wd * 1e3 / args.steps,
+                                     "form": "pyResponseTDI(precision='mixed-c128'): "
+                                             "get_projections + get_tdi_delays per step"}
+
     out_fast = out_fused64 = None
     if args.only is None:
         # the other FP64 GPU forms (rel-L2 <= 1e-12 against the oracle rather than bitwise):

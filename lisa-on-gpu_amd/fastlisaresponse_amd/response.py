@@ -63,6 +63,38 @@ def _as_orbits(orbits) -> Orbits:
     return orbits
 
 
+class _FusedBatches:
+    """One reserved single-source fused batch per flag set, for pyResponseTDI's mixed
+    precisions: reserved on first use, rebound per call. Holds the response handle so that it
+    outlives the batches; destroys them after the device has finished with them."""
+
+    def __init__(self, cpp_response):
+        self.cpp_response = cpp_response
+        self.batches = {}
+
+    def get(self, flags, dt, fs):
+        import ctypes
+        key = (flags, dt, fs)
+        hb = self.batches.get(key)
+        if hb is None:
+            hb = ctypes.c_void_p()
+            _capi.check(_capi.lib().flr_fused_reserve(self.cpp_response._h, 1, dt, fs, flags,
+                                                      ctypes.byref(hb)))
+            self.batches[key] = hb
+        return hb
+
+    def __del__(self):
+        try:
+            if self.batches and _capi._lib is not None:
+                if torch.cuda.is_initialized():
+                    torch.cuda.synchronize()
+                for hb in self.batches.values():
+                    _capi.lib().flr_batch_destroy(hb)
+                self.batches.clear()
+        except Exception:
+            pass
+
+
 class pyResponseTDI:
     """Generic time-domain LISA response: link projections and TDI combinations.
 
@@ -137,6 +169,7 @@ class pyResponseTDI:
         self.delayed_links_flat = None
         self._xyz_stale = False
         self._mixed = None  # precision="mixed": (h / scale as complex64, scale, k, u, v)
+        self._fused = None  # mixed precisions: reserved one-source fused batches (_FusedBatches)
 
     # ------------------------------------------------------------ properties
     @property
@@ -338,17 +371,16 @@ class pyResponseTDI:
             flags |= _capi.FLR_FLAG_C128
         if aet:
             flags |= _capi.FLR_FLAG_AET
-        L = _capi.lib()
-        hb = ctypes.c_void_p()
-        _capi.check(L.flr_fused_create(self.cpp_response._h, srcs, 1, h32.data_ptr(),
-                                       out.data_ptr(), self.dt, self.sampling_frequency, flags,
-                                       ctypes.byref(hb)))
-        try:
-            from .backend import current_stream
-            _capi.check(L.flr_batch_run(hb, current_stream()))
-            self.cpp_response.check_errors()  # synchronises before the source table is freed
-        finally:
-            L.flr_batch_destroy(hb)
+        from .backend import current_stream
+        if self._fused is None:
+            self._fused = _FusedBatches(self.cpp_response)
+        hb = self._fused.get(flags, self.dt, self.sampling_frequency)
+        stream = current_stream()
+        # bind (asynchronous table upload) and run on the caller's stream: no allocation and no
+        # synchronisation per call; window errors are latched for check_errors
+        _capi.check(_capi.lib().flr_fused_bind(hb, srcs, 1, ctypes.c_void_p(h32.data_ptr()),
+                                               ctypes.c_void_p(out.data_ptr()), stream))
+        _capi.check(_capi.lib().flr_batch_run(hb, stream))
         if not aet:
             self.delayed_links_flat = out
             self._xyz_stale = False

@@ -286,6 +286,36 @@ def test_c128_vs_fp64_oracle_at_5mhz(torch_gpu, orbits, oracle_orbits, oracle_mo
         assert r_c <= TOL_C128, (c, r_c)
 
 
+def test_mixed_dropin_reuses_its_batch(torch_gpu, orbits):
+    """pyResponseTDI's mixed forms keep one reserved batch per flag set and rebind it per call:
+    consecutive calls with different inputs, times and channels give the results of fresh
+    instances, and earlier results are not overwritten."""
+    torch = torch_gpu
+    from fastlisaresponse_amd import pyResponseTDI
+    n = 1 << 15
+    hs = [gb_waveform(n, 10.0, 1e-21, f, 1e-17, 0.8, 0.3, 0.5) for f in (3e-3, 7e-3)]
+    for prec in ("mixed-c128", "mixed"):
+        r = pyResponseTDI(0.1, n, orbits=orbits, tdi="2nd generation", tdi_chan="AET",
+                          precision=prec)
+        outs = []
+        for h, t0 in zip(hs, (0.0, 1.0e6)):
+            r.get_projections(h, 1.2, 0.3, t0=t0)
+            outs.append([c.clone() for c in r.get_tdi_delays()] + [r.get_tdi_delays()[0]])
+        r.tdi_chan = "XYZ"
+        xyz = r.get_tdi_delays()[0]
+        for (h, t0), got in zip(zip(hs, (0.0, 1.0e6)), outs):
+            f = pyResponseTDI(0.1, n, orbits=orbits, tdi="2nd generation", tdi_chan="AET",
+                              precision=prec)
+            f.get_projections(h, 1.2, 0.3, t0=t0)
+            exp = f.get_tdi_delays()
+            for c in range(3):
+                assert torch.equal(got[c], exp[c]), (prec, t0, c)
+            assert torch.equal(got[3], exp[0])
+        f.tdi_chan = "XYZ"
+        assert torch.equal(xyz, f.get_tdi_delays()[0]), prec
+        r.check_errors()
+
+
 def test_mixed_far_times_take_the_per_sample_path(torch_gpu, oracle_mod):
     """At t ~ 1e9 s the rounding of the delays can exceed the composite filters' tolerance, so
     tiles are not cleared as a whole and samples are tested one by one; those that fail take
