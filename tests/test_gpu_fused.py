@@ -286,6 +286,33 @@ def test_c128_vs_fp64_oracle_at_5mhz(torch_gpu, orbits, oracle_orbits, oracle_mo
         assert r_c <= TOL_C128, (c, r_c)
 
 
+def test_bound_filters_follow_table_changes(torch_gpu, orbits):
+    """The composite TDI filters are built when sources are bound; replacing the response's
+    TDI table afterwards rebuilds them at the next run (same bits as a plan made for the new
+    table)."""
+    torch = torch_gpu
+    from fastlisaresponse_amd.batch import BatchedResponse
+    from fastlisaresponse_amd.tables import stock_tdi, tdi_tables
+    rng = np.random.default_rng(59)
+    hs, params = _sources(rng, [20000, 9000])
+    br = BatchedResponse(0.1, tdi="2nd generation", orbits=orbits, precision="mixed-c128")
+    plan = _plan(torch, br, hs, params)
+    plan.run()
+    out2 = plan.out_all.clone()
+    tabs = tdi_tables(stock_tdi("1st generation"))
+    br.model.cpp_response.set_tdi(tabs["unit_starts"], tabs["unit_lengths"], tabs["tdi_base_links"],
+                                  tabs["tdi_link_combinations"], tabs["tdi_signs"],
+                                  tabs["channels"], tabs["num_units"], 3)
+    plan.run()
+    plan.check_errors()
+    b1 = BatchedResponse(0.1, tdi="1st generation", orbits=orbits, precision="mixed-c128")
+    p1 = _plan(torch, b1, hs, params)
+    p1.run()
+    p1.check_errors()
+    assert torch.equal(plan.out_all, p1.out_all)
+    assert not torch.equal(plan.out_all, out2)
+
+
 def test_mixed_dropin_reuses_its_batch(torch_gpu, orbits):
     """pyResponseTDI's mixed forms keep one reserved batch per flag set and rebind it per call:
     consecutive calls with different inputs, times and channels give the results of fresh
