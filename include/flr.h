@@ -213,7 +213,7 @@ int flr_batch_run(flr_batch *batch, void *stream);
  * workgroup tile (fused mixed: 256, 512 or 1024; fused FP64: 512; unfused: 256; 0 keeps the
  * current; a fused tile outputs tile_samples minus the TDI delay halos, so that its projection
  * window is whole rounds of the workgroup) and grid_per_cu persistent workgroups per CU
- * (1..64; 0 keeps the current, default 16). Results do not depend on either. Call it when no
+ * (1..64; 0 keeps the current, default 32). Results do not depend on either. Call it when no
  * launch of the batch is in flight; the autotuner (fastlisaresponse_amd.autotune) times the
  * candidates and keeps the fastest. */
 int flr_batch_set_launch(flr_batch *batch, int tile_samples, int grid_per_cu);

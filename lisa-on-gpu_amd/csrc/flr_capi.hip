@@ -151,7 +151,7 @@ struct flr_batch {
     int capacity = 0;             // fused: sources the tables hold
     int tile_passes = kFusedPasses;  // fused: the tile's window is tile_passes * kBlock samples
     int tile_len = 0;                // fused: output samples per tile (make_fused_config)
-    int grid_per_cu = 16;            // persistent workgroups per CU
+    int grid_per_cu = 32;            // persistent workgroups per CU (the autotuner's usual pick)
     FusedSource *h_fused = nullptr;  // pinned staging of the source table and tile prefix
     int *h_fused_tiles = nullptr;
     hipEvent_t uploaded = nullptr;   // the last bind's upload

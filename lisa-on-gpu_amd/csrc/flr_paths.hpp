@@ -220,7 +220,7 @@ inline hipError_t set_lds_limit(Kernel k, size_t bytes)
 
 // Persistent batch grid: grid_per_cu workgroups per CU (enough to fill every CU a few times
 // over), never more than there are tiles (flr_kernels.hip).
-int batch_grid(int total_tiles, int grid_per_cu = 16);
+int batch_grid(int total_tiles, int grid_per_cu = 32);
 
 template <typename Kernel, typename... Args>
 inline hipError_t launch(Kernel k, int nblocks, size_t shmem, hipStream_t stream, Args... args)

@@ -199,7 +199,7 @@ class BatchPlan:
         self._h = h
         self.num_sources = n
         self.samples = int(lib().flr_batch_samples(h))
-        self.launch_shape = (512 if br.fused else 256, 16)  # (tile window samples, WGs per CU)
+        self.launch_shape = (512 if br.fused else 256, 32)  # (tile window samples, WGs per CU)
 
     def tune(self, **kw) -> dict:
         """Pick the fastest launch shape for this plan (fastlisaresponse_amd.autotune.tune)."""
