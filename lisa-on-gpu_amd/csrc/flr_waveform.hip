@@ -6,7 +6,8 @@
 //   h+_s = -cos(phase) A (1 + cos^2 i),  hx_s = -sin(phase) 2 A cos i,
 //   h+ = h+_s cos 2psi - hx_s sin 2psi,  hx = h+_s sin 2psi + hx_s cos 2psi,
 // with t = t0 + k dt, evaluated in FP64 in that operation order (the library is built with
-// -ffp-contract=off). cos 2psi, sin 2psi and cos i come from the host.
+// -ffp-contract=off; the rounding of 2 pi times a phase of many cycles is the reference's too,
+// and the null channel T is sensitive to it). cos 2psi, sin 2psi and cos i come from the host.
 #include "flr_internal.hpp"
 
 namespace flr {
@@ -25,35 +26,48 @@ __device__ __forceinline__ int gb_source_of(const flr_gb_source *s, int n, int64
     return lo;
 }
 
+constexpr int kGbPerThread = 4;  // samples per thread: one source lookup per 1024 samples
+
 __global__ __launch_bounds__(kBlock) void gb_kernel(const flr_gb_source *src, int num_sources,
                                                     int64_t total, double dt, int format,
                                                     void *out)
 {
-    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (g >= total) return;
-    const flr_gb_source s = src[gb_source_of(src, num_sources, g)];
-    const int64_t k = g - s.offset;
-    if (k < 0 || k >= s.num_pts) return;  // a gap between sources: left untouched
-    const double t = s.t0 + (double)k * dt;
-    const double two_pi = 6.283185307179586;  // 2 * pi in FP64
-    const double tt = t * t;
-    const double phase =
-        two_pi * ((s.f0 * t + 0.5 * s.fdot * tt) + (1.0 / 6.0) * s.fddot * (tt * t)) - s.phi0;
-    double sn, cs;
-    sincos(phase, &sn, &cs);
-    const double hSp = -cs * s.amplitude * (1.0 + s.cos_iota * s.cos_iota);
-    const double hSc = -sn * 2.0 * s.amplitude * s.cos_iota;
-    const double hp = hSp * s.cos2psi - hSc * s.sin2psi;
-    const double hc = hSp * s.sin2psi + hSc * s.cos2psi;
-    if (format == FLR_GB_COMPLEX128) {
-        reinterpret_cast<double2 *>(out)[g] = make_double2(hp, hc);
-    } else if (format == FLR_GB_COMPLEX64) {
-        reinterpret_cast<float2 *>(out)[g] = make_float2((float)(hp / s.scale), (float)(hc / s.scale));
-    } else {
-        const double full = 2147483647.0;  // 2^31 - 1
-        const double qr = fmin(fmax(rint(hp / s.scale * full), -full), full);
-        const double qi = fmin(fmax(rint(hc / s.scale * full), -full), full);
-        reinterpret_cast<int2 *>(out)[g] = make_int2((int)qr, (int)qi);
+    // the workgroup's first sample's source, found once; each thread then steps forward from it
+    // (a 1024-sample range spans few sources)
+    __shared__ int first_source;
+    const int64_t g0 = (int64_t)blockIdx.x * (kBlock * kGbPerThread);
+    if (threadIdx.x == 0) first_source = gb_source_of(src, num_sources, g0);
+    __syncthreads();
+    int si = first_source;
+#pragma unroll 1
+    for (int r = 0; r < kGbPerThread; ++r) {
+        const int64_t g = g0 + (int64_t)r * kBlock + threadIdx.x;
+        if (g >= total) return;
+        while (si + 1 < num_sources && src[si + 1].offset <= g) ++si;
+        const flr_gb_source s = src[si];
+        const int64_t k = g - s.offset;
+        if (k < 0 || k >= s.num_pts) continue;  // a gap between sources: left untouched
+        const double t = s.t0 + (double)k * dt;
+        const double two_pi = 6.283185307179586;  // 2 * pi in FP64
+        const double tt = t * t;
+        const double phase =
+            two_pi * ((s.f0 * t + 0.5 * s.fdot * tt) + (1.0 / 6.0) * s.fddot * (tt * t)) - s.phi0;
+        double sn, cs;
+        sincos(phase, &sn, &cs);
+        const double hSp = -cs * s.amplitude * (1.0 + s.cos_iota * s.cos_iota);
+        const double hSc = -sn * 2.0 * s.amplitude * s.cos_iota;
+        const double hp = hSp * s.cos2psi - hSc * s.sin2psi;
+        const double hc = hSp * s.sin2psi + hSc * s.cos2psi;
+        if (format == FLR_GB_COMPLEX128) {
+            reinterpret_cast<double2 *>(out)[g] = make_double2(hp, hc);
+        } else if (format == FLR_GB_COMPLEX64) {
+            reinterpret_cast<float2 *>(out)[g] = make_float2((float)(hp / s.scale), (float)(hc / s.scale));
+        } else {
+            const double full = 2147483647.0;  // 2^31 - 1
+            const double qr = fmin(fmax(rint(hp / s.scale * full), -full), full);
+            const double qi = fmin(fmax(rint(hc / s.scale * full), -full), full);
+            reinterpret_cast<int2 *>(out)[g] = make_int2((int)qr, (int)qi);
+        }
     }
 }
 
@@ -61,7 +75,8 @@ hipError_t launch_gb(const flr_gb_source *sources, int num_sources, int64_t tota
                      int format, void *out, hipStream_t stream)
 {
     if (total <= 0) return hipSuccess;
-    const int64_t blocks = (total + kBlock - 1) / kBlock;
+    const int64_t per_block = (int64_t)kBlock * kGbPerThread;
+    const int64_t blocks = (total + per_block - 1) / per_block;
     hipLaunchKernelGGL(gb_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, stream, sources,
                        num_sources, total, dt, format, out);
     return hipGetLastError();
