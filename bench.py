@@ -247,7 +247,7 @@ def serving_run(seconds, rate, seed=5):
     orb = EqualArmlengthOrbits(t_max=1.5 * YRSID_SI)
     orb.configure(linear_interp_setup=True)
     keys = [(dt, g) for dt in (5.0, 10.0) for g in ("1st generation", "2nd generation")]
-    server = ResponseServer(orb, keys, hbm_budget=64 << 30, num_streams=4)
+    server = ResponseServer(orb, keys, hbm_budget=64 << 30, num_streams=2)
     # warm-up: a few small requests, outside the measurement
     wt, wr = poisson_requests(np.random.default_rng(seed + 1), 50.0, 0.2,
                               lengths=(1 << 18,), max_sources=4)
@@ -264,7 +264,7 @@ def serving_run(seconds, rate, seed=5):
                   "workload": "Poisson arrivals, 1-64 sources/request, N in {2^18, 2^20, 2^22}, "
                               "dt in {5, 10} s, 30/70 1st/2nd gen, galactic-binary parameters "
                               "generated on the device as complex128, fused kernel with "
-                              "float32 outputs, 4 streams, 64 GiB admission budget"})
+                              "float32 outputs, 2 streams, 64 GiB admission budget"})
     return stats
 
 

@@ -118,7 +118,7 @@ class ResponseServer:
     BYTES_PER_SAMPLE = 16 + 12  # per sample in flight: input + float32 A, E, T (default format)
 
     def __init__(self, orbits, keys: Sequence[Tuple[float, str]], hbm_budget: int = 64 << 30,
-                 num_streams: int = 4, max_launch_samples: int = 1 << 27, max_sources: int = 512,
+                 num_streams: int = 2, max_launch_samples: int = 1 << 27, max_sources: int = 512,
                  keep_results: bool = False, t_buffer: float = 10000.0, tdi_chan: str = "AET",
                  precision: str = "mixed-c128"):
         if precision not in self.INPUT_BYTES:
