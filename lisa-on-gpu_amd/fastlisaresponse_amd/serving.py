@@ -25,6 +25,7 @@ whichever requests it shares a launch with (tiles never mix sources).
 from __future__ import annotations
 
 import ctypes
+import os
 import time
 from collections import deque
 from dataclasses import dataclass, field
@@ -123,6 +124,11 @@ class ResponseServer:
                  precision: str = "mixed-c128"):
         if precision not in self.INPUT_BYTES:
             raise ValueError(f"precision must be one of {sorted(self.INPUT_BYTES)}")
+        queues = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
+        if not 1 <= num_streams <= queues:
+            # more streams than hardware queues serialise behind one another (measured: 8
+            # streams on 4 queues served a quarter of the throughput)
+            raise ValueError(f"num_streams must be in [1, {queues}] (hardware queues per process)")
         self.precision = precision
         self.BYTES_PER_SAMPLE = self.INPUT_BYTES[precision] + 12
         self._in_dtype = torch.complex128 if precision == "mixed-c128" else torch.complex64
