@@ -8,7 +8,7 @@ namespace flr {
 
 // ------------------------------------------------------------------ fused projection + TDI
 // One tile = tc.tile output samples, sized on the host so that tile + TDI halos is exactly
-// PASSES * kBlock window samples: the projection phase then runs PASSES full rounds of the
+// PASSES * TPB window samples (TPB threads per workgroup): the projection phase then runs PASSES full rounds of the
 // workgroup and the TDI phase at most PASSES rounds (the last one partly idle). Stage h over the tile's TDI window plus
 // the projection halo (converted to FP64), project the six links of every window sample into
 // LDS (zero outside [proj_start, n - proj_start), as the reference's zeroed y_gw), then run TDI
@@ -131,7 +131,7 @@ __global__ __launch_bounds__(kBlock) void lti_filter_kernel(TdiConfig tc,
 // clamped into the window, so edge blocks read finite samples wherever F is zero; their rows
 // outside the tile, and samples left to the per-sample path (lti_ok = 0 unless all_ok), are
 // not stored.
-template <typename OT>
+template <typename OT, int TPB>
 __device__ __forceinline__ void lti_mfma(const FusedSource &src, int first, int last, int i0,
                                          int yw_lo, int yw_len, int kmin, const double *ywin,
                                          int pitch, const double *fpad,
@@ -144,7 +144,7 @@ __device__ __forceinline__ void lti_mfma(const FusedSource &src, int first, int 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nb = lane & 15, kq = lane >> 4;
     const size_t n = (size_t)src.num_pts;
-    for (int g = wave; g < 3 * ngc; g += kBlock / 64) {
+    for (int g = wave; g < 3 * ngc; g += TPB / 64) {
         const int o = g / ngc;
         const int qb = q0 + 16 * (g - o * ngc) + nb;
         const int wb = T + 16 * min(qb, q1) + kmin - yw_lo;  // window column of tap m = 0
@@ -170,7 +170,7 @@ __device__ __forceinline__ void lti_mfma(const FusedSource &src, int first, int 
     }
 }
 
-template <bool EXACT, int MODE, int NX, typename HT, typename OT, int PASSES>
+template <bool EXACT, int MODE, int NX, typename HT, typename OT, int PASSES, int TPB = kBlock>
 __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig &tc,
                                            const FusedSource &src, int s, int tile,
                                            const OrbitView &orb, const LagrangeView &lg,
@@ -178,7 +178,7 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
 {
     constexpr int H = 13;
     constexpr int kPasses = PASSES;
-    const int tile_len = tc.tile;  // <= PASSES * kBlock
+    const int tile_len = tc.tile;  // <= PASSES * TPB
     const int tid = threadIdx.x;
     const int n = src.num_pts;
     const int T = src.tdi_start;
@@ -188,7 +188,7 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
     if (tc.zero_edges) {
 #pragma unroll
         for (int r = 0; r < kPasses; ++r) {
-            const int i = i0 + r * kBlock + tid;
+            const int i = i0 + r * TPB + tid;
             if (i < n && (i < T || i >= n - T))
                 for (int c = 0; c < 3; ++c) store_out<OT>(src.out, (size_t)c * n + i, 0.0);
         }
@@ -200,7 +200,7 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
     const int yw_len = last + tc.halo_hi - yw_lo + 1;
     if (yw_len > cap || yw_lo < 0 || yw_lo + yw_len > n) {
         for (int r = 0; r < kPasses; ++r) {
-            const int i = i0 + r * kBlock + tid;
+            const int i = i0 + r * TPB + tid;
             if (i < first || i > last) continue;
             atomicOr(tc.err, kErrWindow);
             for (int c = 0; c < 3; ++c) store_out<OT>(src.out, (size_t)c * n + i, __builtin_nan(""));
@@ -218,14 +218,14 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
     double2 *hwin = reinterpret_cast<double2 *>(smem);
     double *ywin = smem + 2 * pc.win_cap;  // [6][pitch]
     double *fpad = ywin + kNLinks * pitch;  // kLti: [2][kFpad]
-    unsigned char *lti_ok = reinterpret_cast<unsigned char *>(fpad + 2 * kFpad);  // [PASSES * kBlock]
-    for (int jj = tid; jj < hw_len; jj += kBlock) hwin[jj] = load_h<HT>(src.h, hw_lo + jj);
+    unsigned char *lti_ok = reinterpret_cast<unsigned char *>(fpad + 2 * kFpad);  // [PASSES * TPB]
+    for (int jj = tid; jj < hw_len; jj += TPB) hwin[jj] = load_h<HT>(src.h, hw_lo + jj);
     __syncthreads();
 
     const double t0_offset = src.t0 + (double)0 * pc.dt;
     const double y_deps = EXACT ? rcp_nr2(lg.deps) : 0.0;
     bool bad = false;
-    for (int w = tid; w < yw_len; w += kBlock) {
+    for (int w = tid; w < yw_len; w += TPB) {
         const int j = yw_lo + w;
         int col = kLti ? lti_col(w) : w;
         pin(col);  // an opaque column: the padded index is not re-derived inside the projection
@@ -251,7 +251,7 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
         tile_ok = lti && lti_tile_matches<NX>(tc, src, orb, rm, lref, L_ref, first, last);
         const double *fir = tc.lti_fir + (size_t)s * (2 * kFir);
         if (lti) {  // read by lti_mfma after the barrier that follows the per-sample pass
-            for (int x = tid; x < 2 * kFpad; x += kBlock) {
+            for (int x = tid; x < 2 * kFpad; x += TPB) {
                 const int kind = x / kFpad, k = x - kind * kFpad - kFpadLo;
                 fpad[x] = (k >= 0 && k < kFir) ? fir[kind * kFir + k] : 0.0;
             }
@@ -260,7 +260,7 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
 
 #pragma unroll 1
     for (int r = 0; r < (tile_ok ? 0 : kPasses); ++r) {  // tile_ok: every sample goes below
-        const int i = i0 + r * kBlock + tid;
+        const int i = i0 + r * TPB + tid;
         if (i < first || i > last) continue;
         const double t = src.t0 + (double)i * tc.dt;
         double acc[3] = {0.0, 0.0, 0.0};
@@ -277,6 +277,7 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
             continue;
         }
         if constexpr (MODE == kModeGeneric) {
+            static_assert(TPB == kBlock, "the generic TDI scratch has kBlock columns");
             double *Ls = ywin + kNLinks * cap;  // [6][kBlock], one column per lane
             const OrbitNode nd = orbit_node<EXACT>(orb, t);
 #pragma unroll
@@ -310,7 +311,7 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
     if constexpr (kLti) {
         if (lti) {
             __syncthreads();  // lti_ok written
-            lti_mfma<OT>(src, first, last, i0, yw_lo, yw_len, lref.kmin, ywin, pitch, fpad, lti_ok,
+            lti_mfma<OT, TPB>(src, first, last, i0, yw_lo, yw_len, lref.kmin, ywin, pitch, fpad, lti_ok,
                          tile_ok);
         }
     }
@@ -329,13 +330,15 @@ __global__ __launch_bounds__(kBlock) void fused_batch_kernel(ProjConfig pc, TdiC
 
 // The mixed equal-arm form (composite filters on the matrix cores), held to 4 waves per SIMD:
 // its VGPRs and the matrix accumulator share one 512-entry register file per lane.
-template <int NX, typename HT, int PASSES>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
+// TPB threads per workgroup: 256, or 512 for the 1024-sample window (two rounds of 512 lanes;
+// its 72 KB of LDS then still leave 16 waves per CU, with half the halo recompute of 512).
+template <int NX, typename HT, int PASSES, int TPB>
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4)))
 void fused_lti_kernel(ProjConfig pc, TdiConfig tc, Batch<FusedSource> batch, OrbitView orb,
                       LagrangeView lg, RowMap rm)
 {
     extern __shared__ double smem[];
-    FLR_BATCH_LOOP(batch, (fused_tile<false, kModeLinear, NX, HT, float, PASSES>(pc, tc, src, s_, tile, orb, lg, rm, smem)));
+    FLR_BATCH_LOOP(batch, (fused_tile<false, kModeLinear, NX, HT, float, PASSES, TPB>(pc, tc, src, s_, tile, orb, lg, rm, smem)));
 }
 
 // Launch one fused variant over a batch: the TDI mode follows the table (stock + equal arms:
@@ -356,9 +359,12 @@ hipError_t launch_fused_variant(const ProjConfig &pc, const TdiConfig &tc, const
         if constexpr (kEq == kModeLinear && std::is_same<OT, float>::value) {
             // tc.lti_fir holds the sources' composite filters (launch_lti_filters)
             if (!tc.lti_fir) return hipErrorInvalidValue;
+            // the 1024-sample window runs as two rounds of 512 lanes
+            constexpr int TPB = PASSES == 4 ? 2 * kBlock : kBlock;
+            constexpr int P = PASSES == 4 ? 2 : PASSES;
             return st.generation == 1
-                       ? launch(fused_lti_kernel<8, HT, PASSES>, nb, shm, stream, pc, tc, b, orb, lg, rm)
-                       : launch(fused_lti_kernel<16, HT, PASSES>, nb, shm, stream, pc, tc, b, orb, lg, rm);
+                       ? launch_tpb(fused_lti_kernel<8, HT, P, TPB>, nb, TPB, shm, stream, pc, tc, b, orb, lg, rm)
+                       : launch_tpb(fused_lti_kernel<16, HT, P, TPB>, nb, TPB, shm, stream, pc, tc, b, orb, lg, rm);
         } else {
             return st.generation == 1 ? FLR_FUSED(kEq, 8) : FLR_FUSED(kEq, 16);
         }

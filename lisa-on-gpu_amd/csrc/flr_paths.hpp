@@ -223,6 +223,17 @@ inline hipError_t set_lds_limit(Kernel k, size_t bytes)
 int batch_grid(int total_tiles, int grid_per_cu = 32);
 
 template <typename Kernel, typename... Args>
+inline hipError_t launch_tpb(Kernel k, int nblocks, int threads, size_t shmem, hipStream_t stream,
+                             Args... args)
+{
+    if (nblocks <= 0) return hipSuccess;
+    hipError_t err = set_lds_limit(k, shmem);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(k, dim3(nblocks), dim3(threads), shmem, stream, args...);
+    return hipGetLastError();
+}
+
+template <typename Kernel, typename... Args>
 inline hipError_t launch(Kernel k, int nblocks, size_t shmem, hipStream_t stream, Args... args)
 {
     if (nblocks <= 0) return hipSuccess;
