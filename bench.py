@@ -30,7 +30,8 @@ Also on the same JSON line:
                 1..64-source requests (2^18/2^20/2^22 samples, dt 5/10 s, 30/70 1st/2nd gen)
   parity        rel-L2 / max-abs of A, E, T against the FP64 oracle on the full workload
   cpu_baseline  the FP64 CPU oracle (1 thread, the reference CPU path's serial arithmetic) on
-                a bounded prefix of the same workload
+                a bounded prefix of the same workload; `all_cores`: the same oracle with OpenMP
+                over samples (up to 16 threads, the host share of one GPU) on the whole source
 """
 from __future__ import annotations
 
@@ -96,8 +97,10 @@ def pmc_traffic(name: str, variant: str = ""):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(orbits, h, lam, beta, tdi, n_sample):
-    """Oracle projection + TDI + AET on the first n_sample samples, one thread."""
+def cpu_baseline(orbits, h, lam, beta, tdi, n_sample, max_cores=16):
+    """Oracle projection + TDI + AET on the first n_sample samples, one thread (the reference
+    CPU path is serial); beside it, the same oracle over samples on up to max_cores threads
+    on the whole source."""
     from oracle import oracle as orc
     oo = orc.OracleOrbits(*orbits.pycppdetector_args)
     import numpy as np
@@ -105,9 +108,20 @@ def cpu_baseline(orbits, h, lam, beta, tdi, n_sample):
     t0 = time.perf_counter()
     orc.response_pipeline(oo, hs, lam, beta, DT, ORDER, tdi, x_base=orbits.x_base, nthreads=1)
     el = time.perf_counter() - t0
-    return {"value": n_sample / el, "unit": "samples/s", "cores": 1, "kind": "port",
-            "sample": f"first {n_sample} samples of the same source, projection + {tdi} TDI + AET "
-                      f"({el:.1f} s, FP64 oracle restating the reference CPU path)"}
+    out = {"value": n_sample / el, "unit": "samples/s", "cores": 1, "kind": "port",
+           "sample": f"first {n_sample} samples of the same source, projection + {tdi} TDI + AET "
+                     f"({el:.1f} s, FP64 oracle restating the reference CPU path)"}
+    threads = max(1, min(max_cores, orc.max_threads(), os.cpu_count() or 1))
+    if threads > 1:
+        hf = np.ascontiguousarray(h)
+        t0 = time.perf_counter()
+        orc.response_pipeline(oo, hf, lam, beta, DT, ORDER, tdi, x_base=orbits.x_base,
+                              nthreads=threads)
+        el = time.perf_counter() - t0
+        out["all_cores"] = {"value": len(hf) / el, "threads": threads,
+                            "sample": f"the whole {len(hf)}-sample source, OpenMP over samples "
+                                      f"({el:.1f} s)"}
+    return out
 
 
 def timed(fn, steps, warmup):
