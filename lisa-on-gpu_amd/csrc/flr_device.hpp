@@ -2,10 +2,13 @@
 //
 // Everything that feeds a time, a delay or the integer/fraction index split repeats the
 // reference CPU operation sequence exactly (this file is compiled with -ffp-contract=off, so
-// no multiply-add is fused behind our back). The stencils come in two forms:
+// no multiply-add is fused behind our back); the fast forms only pick the orbit-table window by
+// a multiplication (the interpolant is continuous across nodes). The stencils and the link
+// geometry come in two forms:
 //   exact: the reference's operation sequence with correctly rounded divisions -- bitwise
 //          equal to the FP64 CPU path;
-//   fast:  the same interpolant without per-tap divisions, equal to ~1e-15 relative.
+//   fast:  the same interpolant without per-tap divisions, and explicit fused multiply-adds in
+//          the link geometry, equal to ~1e-15 relative.
 #pragma once
 
 #include "flr_internal.hpp"
@@ -79,6 +82,18 @@ __device__ __forceinline__ double dot3(const double *a, const double *b)
     out += a[1] * b[1];
     out += a[2] * b[2];
     return out;
+}
+
+// The fast forms' link geometry (unit vector, projections): the same dot product with fused
+// multiply-adds (equal to the exact form up to rounding). Arrival times t - (k.x)/c keep the
+// exact form in both: at t ~ 1e7 s one rounding step of t moves the signal by ~1e-10 of itself.
+template <bool EXACT>
+__device__ __forceinline__ double dot3_form(const double *a, const double *b)
+{
+    if constexpr (EXACT)
+        return dot3(a, b);
+    else
+        return __builtin_fma(a[2], b[2], __builtin_fma(a[1], b[1], a[0] * b[0]));
 }
 
 // Integer/fraction split of a delay (LISAResponse.cu:339-341, :616-622): the stencil sits at
@@ -236,14 +251,15 @@ __device__ __forceinline__ double2 stencil_fast_c(const double2 *x, double e, co
     for (int j = 1; j < h; j += 1) {
         double2 up = x[1 + j];
         double2 down = x[-j];
-        double qu = (double)j + e;
-        double qd = (double)(j + 1) - e;
         double Q = (double)(j * (j + 1)) + D;
         double EP = lg.E[j - 1] * P;
-        double nr = __builtin_fma(up.x, qd, down.x * qu);
-        double ni = __builtin_fma(up.y, qd, down.y * qu);
-        Tr = __builtin_fma(Tr, Q, nr * EP);
-        Ti = __builtin_fma(Ti, Q, ni * EP);
+        double eEP = e * EP;
+        // the pair's weights times E_{j-1} P_{j-1}, formed once for both components:
+        // (j+1-e) EP for the upper tap, (j+e) EP for the lower
+        double wu = __builtin_fma((double)(j + 1), EP, -eEP);
+        double wd = __builtin_fma((double)j, EP, eEP);
+        Tr = __builtin_fma(Tr, Q, __builtin_fma(up.x, wu, down.x * wd));
+        Ti = __builtin_fma(Ti, Q, __builtin_fma(up.y, wu, down.y * wd));
         P = P * Q;
         // one tap pair in registers at a time (the loads of the next j are not hoisted): keeps
         // the fused kernels at 4 waves/SIMD

@@ -101,11 +101,17 @@ __device__ __forceinline__ bool project_links(const ProjConfig &cfg, const doubl
                 if (!bq) he = stencil_c<EXACT, H>(hwin + le, e_em, lg, y_deps);
             }
             double norm = 0.0;
+            if constexpr (EXACT) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                double nt = xr[c] - xe[c];
-                n[c] = nt;
-                norm += nt * nt;
+                for (int c = 0; c < 3; ++c) {
+                    double nt = xr[c] - xe[c];
+                    n[c] = nt;
+                    norm += nt * nt;
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) n[c] = xr[c] - xe[c];
+                norm = dot3_form<false>(n, n);
             }
             if constexpr (EXACT) {
                 norm = sqrt(norm);
@@ -116,15 +122,18 @@ __device__ __forceinline__ bool project_links(const ProjConfig &cfg, const doubl
 #pragma unroll
                 for (int c = 0; c < 3; ++c) n[c] = n[c] * inv;
             }
-            const double un = dot3(u, n);
-            const double vn = dot3(v, n);
-            const double xi_p = 0.5 * ((un * un) - (vn * vn));
+            const double un = dot3_form<EXACT>(u, n);
+            const double vn = dot3_form<EXACT>(v, n);
+            const double xi_p = EXACT ? 0.5 * ((un * un) - (vn * vn))
+                                      : 0.5 * __builtin_fma(un, un, -(vn * vn));
             const double xi_c = un * vn;
-            const double kn = dot3(k, n);
+            const double kn = dot3_form<EXACT>(k, n);
             double out = __builtin_nan("");
             if (!bq) {
                 const double pre = EXACT ? 1. / (1. - kn) : rcp_nr2(1. - kn);
-                const double large = (he.x - hr.x) * xi_p + (he.y - hr.y) * xi_c;
+                const double large =
+                    EXACT ? (he.x - hr.x) * xi_p + (he.y - hr.y) * xi_c
+                          : __builtin_fma(he.x - hr.x, xi_p, (he.y - hr.y) * xi_c);
                 out = pre * large;
             }
             bad |= bq;
