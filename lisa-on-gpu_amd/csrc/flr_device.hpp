@@ -199,7 +199,8 @@ __device__ __forceinline__ double stencil_fast(const double *x, double e, const 
         double qd = (double)(j + 1) - e;
         double Q = (double)(j * (j + 1)) + D;
         double num = __builtin_fma(up, qd, down * qu);
-        T = __builtin_fma(T, Q, num * (lg.E[j - 1] * P));
+        const double term = num * (lg.E[j - 1] * P);
+        T = j == 1 ? term : __builtin_fma(T, Q, term);  // T_1 = E_0 n_1 (T_0 = 0, P_0 = 1)
         P = P * Q;
     }
     return A * __builtin_fma(D, T * rcp_nr2(P), __builtin_fma(B, x[1], e * x[0]));
@@ -258,8 +259,15 @@ __device__ __forceinline__ double2 stencil_fast_c(const double2 *x, double e, co
         // (j+1-e) EP for the upper tap, (j+e) EP for the lower
         double wu = __builtin_fma((double)(j + 1), EP, -eEP);
         double wd = __builtin_fma((double)j, EP, eEP);
-        Tr = __builtin_fma(Tr, Q, __builtin_fma(up.x, wu, down.x * wd));
-        Ti = __builtin_fma(Ti, Q, __builtin_fma(up.y, wu, down.y * wd));
+        const double nr = __builtin_fma(up.x, wu, down.x * wd);
+        const double ni = __builtin_fma(up.y, wu, down.y * wd);
+        if (j == 1) {  // T_1 = n_1 (T_0 = 0, P_0 = 1)
+            Tr = nr;
+            Ti = ni;
+        } else {
+            Tr = __builtin_fma(Tr, Q, nr);
+            Ti = __builtin_fma(Ti, Q, ni);
+        }
         P = P * Q;
         // one tap pair in registers at a time (the loads of the next j are not hoisted): keeps
         // the fused kernels at 4 waves/SIMD

@@ -319,7 +319,8 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
 }
 
 template <bool EXACT, int MODE, int NX, typename HT, typename OT, int PASSES>
-__global__ __launch_bounds__(kBlock) void fused_batch_kernel(ProjConfig pc, TdiConfig tc,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
+void fused_batch_kernel(ProjConfig pc, TdiConfig tc,
                                                              Batch<FusedSource> batch,
                                                              OrbitView orb, LagrangeView lg,
                                                              RowMap rm)

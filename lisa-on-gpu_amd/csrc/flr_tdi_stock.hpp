@@ -341,17 +341,20 @@ __device__ __forceinline__ bool tdi_linear_cols(double t, double L, double t0_of
         const Taps xr[3] = {cols(0, kind, c0), cols(1, kind, c0), cols(2, kind, c0)};
         double S[3] = {0.0, 0.0, 0.0};
         double P = 1.0;
+        const double D = e * (1.0 - e);
 #pragma unroll
         for (int j = 1; j < H; ++j) {
-            const double qu = (double)j + e;
-            const double qd = (double)(j + 1) - e;
-            const double Q = (double)(j * (j + 1)) + e * (1.0 - e);
+            const double Q = (double)(j * (j + 1)) + D;
             const double EP = lg.E[j - 1] * P;
+            const double eEP = e * EP;
+            // the pair's weights times E_{j-1} P_{j-1}, shared by the three outputs
+            const double wu = __builtin_fma((double)(j + 1), EP, -eEP);
+            const double wd = __builtin_fma((double)j, EP, eEP);
 #pragma unroll
             for (int o = 0; o < 3; ++o) {
                 const Taps &x = xr[o];
-                const double num = __builtin_fma(x[H + j], qd, x[H - 1 - j] * qu);
-                S[o] = __builtin_fma(S[o], Q, num * EP);
+                const double num = __builtin_fma(x[H + j], wu, x[H - 1 - j] * wd);
+                S[o] = j == 1 ? num : __builtin_fma(S[o], Q, num);  // S_1 = n_1
                 pin(S[o]);
             }
             P = P * Q;
@@ -360,7 +363,6 @@ __device__ __forceinline__ bool tdi_linear_cols(double t, double L, double t0_of
         }
         const double A = lagrange_A_fast(lg, e);
         const double B = 1.0 - e;
-        const double D = e * B;
         const double invP = 1.0 / P;
         constexpr double sgn = (double)stock::group_sign(m);
 #pragma unroll
