@@ -155,7 +155,7 @@ struct flr_batch {
     FusedSource *h_fused = nullptr;  // pinned staging of the source table and tile prefix
     int *h_fused_tiles = nullptr;
     hipEvent_t uploaded = nullptr;   // the last bind's upload
-    double *d_lti = nullptr;         // mixed fused: [capacity][2][kFir] composite TDI filters
+    double *d_lti = nullptr;         // mixed fused: [capacity][kLtiRecord] composite TDI filters
     uint64_t lti_version = ~0ull;    // response version the filters were built for (none yet)
 };
 
@@ -895,7 +895,7 @@ int flr_fused_reserve(flr_response *r, int capacity, double dt, double sampling_
     if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_fused_tiles, sizeof(int) * (capacity + 1));
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->uploaded, hipEventDisableTiming);
     if (e == hipSuccess && (flags & FLR_FLAG_MIXED))
-        e = hipMalloc(&b->d_lti, sizeof(double) * 2 * kFir * capacity);
+        e = hipMalloc(&b->d_lti, sizeof(double) * kLtiRecord * capacity);
     if (e != hipSuccess) {
         flr_batch_destroy(b);
         return hip_fail(e, "fused batch reserve");

@@ -76,7 +76,8 @@ __device__ __forceinline__ double lti_source_reference(const TdiConfig &tc, cons
 __device__ __forceinline__ double ulp_of(double x) { return ldexp(1.0, ilogb(x) - 52); }
 
 // True when lti_matches holds for every sample of [first, last], so that the per-sample test
-// can be skipped (the outputs are then the same bits): (1) the filter's taps fit the window at
+// can be skipped (the outputs are then the same bits); kmin and L_ref are the source's
+// (lti_source_reference): (1) the filter's taps fit the window at
 // the tile's two ends, hence everywhere; (2) the arm length is L_ref, with zero slope, at every
 // orbit node the tile's times fall in, so each sample's chain delays t - m L differ from the
 // reference's only by rounding; (3) that rounding cannot move a position by kLtiTol: per
@@ -85,11 +86,10 @@ __device__ __forceinline__ double ulp_of(double x) { return ldexp(1.0, ilogb(x) 
 // by r, which moves sample i by (i - T) r against the reference.
 template <int NX>
 __device__ __forceinline__ bool lti_tile_matches(const TdiConfig &tc, const FusedSource &src,
-                                                 const OrbitView &orb, const RowMap &rm,
-                                                 const LtiRef<NX> &ref, double L_ref, int first,
-                                                 int last)
+                                                 const OrbitView &orb, const RowMap &rm, int kmin,
+                                                 double L_ref, int first, int last)
 {
-    if (ref.kmin + tc.halo_lo < 0 || ref.kmin + kFir > tc.halo_hi + 1) return false;
+    if (kmin + tc.halo_lo < 0 || kmin + kFir > tc.halo_hi + 1) return false;
     const int row = rm.row[0];
     const double *pa = orbit_node<false>(orb, src.t0 + (double)first * tc.dt).p;
     const double *pb = orbit_node<false>(orb, src.t0 + (double)last * tc.dt).p;
@@ -103,8 +103,9 @@ __device__ __forceinline__ bool lti_tile_matches(const TdiConfig &tc, const Fuse
     return 2.0 * per_sample + drift + 1e-13 <= kLtiTol;
 }
 
-// One workgroup per source: the two composite filters of the equal-arm stock TDI
-// ([2][kFir] doubles per source in tc.lti_fir; unused when the reference does not fit).
+// One workgroup per source: the two composite filters of the equal-arm stock TDI and the
+// reference they are built from (a kLtiRecord record per source in tc.lti_fir; the filters are
+// zero when the reference does not fit).
 template <int NX>
 __global__ __launch_bounds__(kBlock) void lti_filter_kernel(TdiConfig tc,
                                                             Batch<FusedSource> batch,
@@ -112,12 +113,21 @@ __global__ __launch_bounds__(kBlock) void lti_filter_kernel(TdiConfig tc,
                                                             RowMap rm)
 {
     const int s = blockIdx.x;
-    if (s >= batch.num_sources || threadIdx.x >= 2 * kFir) return;
+    const int x = threadIdx.x;
+    if (s >= batch.num_sources || x >= kLtiRecord) return;
     const FusedSource src = batch.sources[s];
     LtiRef<NX> lref;
-    lti_source_reference<NX, false>(tc, src, orb, rm, lref);
-    tc.lti_fir[(size_t)s * (2 * kFir) + threadIdx.x] =
-        lref.ok ? lti_filter_entry<NX>(lref, lg, threadIdx.x) : 0.0;
+    const double L_ref = lti_source_reference<NX, false>(tc, src, orb, rm, lref);
+    double v = 0.0;
+    if (x < 2 * kFir)
+        v = lref.ok ? lti_filter_entry<NX>(lref, lg, x) : 0.0;
+    else if (x == kLtiLref)
+        v = L_ref;
+    else if (x == kLtiKmin)
+        v = (double)lref.kmin;
+    else if (x == kLtiOk)
+        v = lref.ok ? 1.0 : 0.0;
+    tc.lti_fir[(size_t)s * kLtiRecord + x] = v;
 }
 
 // Composite-filter TDI of one tile on the matrix cores. Outputs are taken in blocks of 16
@@ -246,10 +256,14 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
     LtiRef<NX> lref;  // set up after the projection: not live across it
     bool lti = false, tile_ok = false;
     if constexpr (kLti) {
-        const double L_ref = lti_source_reference<NX, EXACT>(tc, src, orb, rm, lref);
-        lti = lref.ok;
-        tile_ok = lti && lti_tile_matches<NX>(tc, src, orb, rm, lref, L_ref, first, last);
-        const double *fir = tc.lti_fir + (size_t)s * (2 * kFir);
+        // the source's reference, from its filter record (lti_filter_kernel); the chain
+        // positions themselves are only needed when the per-sample test runs
+        const double *fir = tc.lti_fir + (size_t)s * kLtiRecord;
+        const double L_ref = fir[kLtiLref];
+        lref.kmin = (int)fir[kLtiKmin];
+        lti = fir[kLtiOk] != 0.0;
+        tile_ok = lti && lti_tile_matches<NX>(tc, src, orb, rm, lref.kmin, L_ref, first, last);
+        if (lti && !tile_ok) (void)lti_source_reference<NX, EXACT>(tc, src, orb, rm, lref);
         if (lti) {  // read by lti_mfma after the barrier that follows the per-sample pass
             for (int x = tid; x < 2 * kFpad; x += TPB) {
                 const int kind = x / kFpad, k = x - kind * kFpad - kFpadLo;

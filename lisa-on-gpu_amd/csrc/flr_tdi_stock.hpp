@@ -424,12 +424,19 @@ __device__ __forceinline__ bool tdi_linear_padded(double t, double L, double t0_
 // filter: the position of chain length m relative to sample i, delta_m = d_m - i + 1 - e_m
 // (the stencil interpolates at sample d_m + 1 - e_m), is the same for every i up to FP64
 // rounding of the times (~1e-9 samples). The groups of one series kind then add up to one
-// composite filter of at most kFir taps, built once per tile from a reference sample;
+// composite filter of at most kFir taps, built once per source from a reference sample;
 // each sample checks that its own positions agree within kLtiTol samples (and falls back to
 // tdi_linear otherwise), so the result differs from per-sample weights by at most
 // kLtiTol * |dx/dsample| per stencil -- used by the mixed-precision kernels only.
 constexpr int kFir = 40;
 constexpr double kLtiTol = 1e-8;
+// Per-source record of a mixed batch's composite filters (tc.lti_fir): the two filters
+// ([2][kFir]), then the reference arm length, the first tap kmin and the fit flag (1 or 0) of
+// lti_source_reference, padded to a multiple of 4 doubles.
+constexpr int kLtiLref = 2 * kFir;
+constexpr int kLtiKmin = 2 * kFir + 1;
+constexpr int kLtiOk = 2 * kFir + 2;
+constexpr int kLtiRecord = 2 * kFir + 4;
 // One kind's composite filter as the matrix-core path reads it: taps -15 .. 55 (zero outside
 // 0 .. kFir-1), so that a 16-output block's Toeplitz operand F[q - r] (q in 0..55, r in 0..15)
 // is one indexed load.
