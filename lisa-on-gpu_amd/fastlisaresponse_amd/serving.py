@@ -113,6 +113,9 @@ class ResponseServer:
         t_buffer: edge length (s) left out of TDI, as pyResponseTDI.get_projections.
         precision: "mixed-c128" (complex128 inputs, float32 outputs; default) or "mixed"
             (complex64 inputs h / scale: half the input memory, FP32 storage error).
+        launch: (tile window samples, workgroups per CU) of every launch; the default 1024-sample
+            window is the autotuner's pick for long sources. A key whose delay windows do not
+            fit that window in LDS (very fine sampling) keeps the library's default shape.
     """
 
     INPUT_BYTES = {"mixed": 8, "mixed-c128": 16}
@@ -121,7 +124,7 @@ class ResponseServer:
     def __init__(self, orbits, keys: Sequence[Tuple[float, str]], hbm_budget: int = 64 << 30,
                  num_streams: int = 2, max_launch_samples: int = 1 << 27, max_sources: int = 512,
                  keep_results: bool = False, t_buffer: float = 10000.0, tdi_chan: str = "AET",
-                 precision: str = "mixed-c128"):
+                 precision: str = "mixed-c128", launch: Optional[Tuple[int, int]] = (1024, 32)):
         if precision not in self.INPUT_BYTES:
             raise ValueError(f"precision must be one of {sorted(self.INPUT_BYTES)}")
         queues = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
@@ -130,6 +133,7 @@ class ResponseServer:
             # streams on 4 queues served a quarter of the throughput)
             raise ValueError(f"num_streams must be in [1, {queues}] (hardware queues per process)")
         self.precision = precision
+        self.launch = None if launch is None else (int(launch[0]), int(launch[1]))
         self.BYTES_PER_SAMPLE = self.INPUT_BYTES[precision] + 12
         self._in_dtype = torch.complex128 if precision == "mixed-c128" else torch.complex64
         self.models: Dict[Tuple[float, str], BatchedResponse] = {}
@@ -161,6 +165,9 @@ class ResponseServer:
             b = ctypes.c_void_p()
             check(lib().flr_fused_reserve(br.model.cpp_response._h, self.max_sources, br.dt,
                                           br.sampling_frequency, flags, ctypes.byref(b)))
+            if self.launch is not None:
+                # a shape the key's windows do not fit is refused: keep the default one
+                lib().flr_batch_set_launch(b, self.launch[0], self.launch[1])
             self._batches[(slot, key)] = b
         return b
 
