@@ -245,11 +245,15 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
             for (int r = 0; r < kNLinks; ++r) yc[r * pitch] = 0.0;
         } else {
             const double t = src.t0 + (double)j * pc.dt;
-            bad |= project_links<EXACT, H, MODE == kModeGroups || MODE == kModeLinear>(pc, src.k, src.u, src.v, t, t0_offset, orb, lg, hwin,
-                                           hw_lo, hw_len, y_deps,
-                                           [&](int row, double val) { yc[row * pitch] = val; });
+            // linear mode: the link rows are only read back by linear_series just below, so they
+            // are kept in canonical link order (compile-time rows)
+            bad |= project_links<EXACT, H, MODE == kModeGroups || MODE == kModeLinear,
+                                 MODE == kModeLinear>(pc, src.k, src.u, src.v, t, t0_offset, orb, lg,
+                                                      hwin, hw_lo, hw_len, y_deps,
+                                                      [&](int row, double val) { yc[row * pitch] = val; });
         }
-        if constexpr (MODE == kModeLinear) linear_series(yc, pitch, rm, tc.aet, tc.s2, tc.s6, tc.s3);
+        if constexpr (MODE == kModeLinear)
+            linear_series<true>(yc, pitch, rm, tc.aet, tc.s2, tc.s6, tc.s3);
     }
     __syncthreads();
 

@@ -31,10 +31,11 @@ __device__ __forceinline__ TileRange tile_range(int tile, int num_delays, int st
 
 // Projections of all six links at time t from the staged h window (hwin[0] = sample win_lo):
 // per receiving spacecraft its stencil (shared by its two links), then each link's emitter
-// geometry and stencil; store(row, value) receives y of table row `row`. Operation order per
-// link follows LISAResponse.cu:559-668. Returns true if a stencil fell outside the window (its
-// links get NaN).
-template <bool EXACT, int H, bool EQ, typename Store>
+// geometry and stencil; store(row, value) receives y of table row `row` (CANON: of the link's
+// canonical index stock::canon, a compile-time row). Operation order per link follows
+// LISAResponse.cu:559-668. Returns true if a stencil fell outside the window (its links get
+// NaN).
+template <bool EXACT, int H, bool EQ, bool CANON = false, typename Store>
 __device__ __forceinline__ bool project_links(const ProjConfig &cfg, const double (&k)[3],
                                               const double (&u)[3], const double (&v)[3],
                                               double t, double t0_offset, const OrbitView &orb,
@@ -79,7 +80,7 @@ __device__ __forceinline__ bool project_links(const ProjConfig &cfg, const doubl
         if (!br) hr = stencil_c<EXACT, H>(hwin + lr, e_rec, lg, y_deps);
         static_for<2>([&](auto q) {
             constexpr int se = (s + 1 + q) % 3;  // the emitters of receiver s
-            const int row = cfg.link_row[s][se];
+            const int row = CANON ? stock::canon(10 * (s + 1) + se + 1) : cfg.link_row[s][se];
             double xe[3], n[3];
             double2 he = make_double2(0.0, 0.0);
             bool bq;

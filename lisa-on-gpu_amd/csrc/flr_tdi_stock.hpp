@@ -285,13 +285,15 @@ __device__ __forceinline__ bool tdi_stock_unequal(double t, const double (&L)[6]
 
 // Column of the combined series (in place of the six y rows, same stride `cap`): row
 // 2*o + kind holds output o's combination of kind `kind` -- X, Y, Z themselves, or A, E, T
-// formed from them when aet is set. yc points at the column; rm maps canonical links to rows.
+// formed from them when aet is set. yc points at the column; rm maps canonical links to rows
+// (CANON: the rows are already in canonical order).
+template <bool CANON = false>
 __device__ __forceinline__ void linear_series(double *yc, int cap, const RowMap &rm, bool aet,
                                               double s2, double s6, double s3)
 {
     double y[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) y[k] = yc[rm.row[k] * cap];
+    for (int k = 0; k < 6; ++k) y[k] = yc[(CANON ? k : rm.row[k]) * cap];
     double kd[2][3];
     static_for<3>([&](auto c) {
         kd[0][c] = y[stock::canon(stock::cyc(13, c))] - y[stock::canon(stock::cyc(12, c))];
