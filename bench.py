@@ -314,8 +314,8 @@ def main() -> int:
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); sources are "
-              "independent and a long source shards by time range with halo recompute, but no "
-              "multi-GPU run is part of this build.", file=sys.stderr)
+              "independent and a long source shards by time range with halo recompute, but the "
+              "multi-GPU path is not implemented.", file=sys.stderr)
         return 2
 
     import numpy as np
