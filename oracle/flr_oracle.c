@@ -16,8 +16,11 @@
  * Orbit interpolation restates the lisatools `Orbits` device class (third-party package
  * lisaanalysistools >= 1.2.8, not present here; call sites LISAResponse.cu:311,326,331,565,
  * 570,576): uniform-grid linear interpolation of the tabulated light-travel times [N][6]
- * and spacecraft positions [N][3][3]. Its exact lisatools formula cannot be checked offline;
- * the tests pin it against numpy.interp instead.
+ * and spacecraft positions [N][3][3]. Its exact lisatools formula cannot be checked offline:
+ * parity unpinned against lisatools (the tests pin it against numpy.interp instead). The XYZ ->
+ * AET convention (A = (Z - X)/sqrt 2, E = (X - 2Y + Z)/sqrt 6, T = (X + Y + Z)/sqrt 3) is
+ * likewise parity unpinned against lisatools; the tests pin its orthonormality and the images
+ * of unit inputs.
  *
  * Build with -ffp-contract=off: the reference's operation sequence has no fused multiply-adds
  * and the time arithmetic must round exactly as written.

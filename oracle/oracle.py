@@ -13,6 +13,11 @@ src/fastlisaresponse/response.py:
     buffers           <- get_projections buffer arithmetic         response.py:474-500
     sky_basis         <- get_projections k, u, v                   response.py:503-523
     unit_ranges       <- get_tdi_delays np.unique                  response.py:619-633
+
+Parity unpinned (third-party lisaanalysistools is not in the image): the lisatools Orbits
+interpolation formula (pinned against numpy.interp instead) and the lisatools AET convention
+(pinned by orthonormality and the images of unit inputs). Everything else is checked against
+closed forms, invariants and brute-force sums in tests/test_oracle_pinning.py.
 """
 from __future__ import annotations
 
