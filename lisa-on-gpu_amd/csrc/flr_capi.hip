@@ -974,6 +974,13 @@ int flr_fused_bind(flr_batch *b, const flr_source *sources, int num_sources, con
     }
     hipStream_t st = (hipStream_t)stream;
     FLR_HIP(hipEventSynchronize(b->uploaded));  // the staging buffers are free again
+    // A source's composite filters depend on its first time and first TDI sample (and on the
+    // response's tables, tracked by lti_version), not on its sky position or input: when these
+    // match the previous bind's (still in the staging copy), the filters built then -- already
+    // complete, their launch precedes the `uploaded` event waited on above -- are kept.
+    bool same_refs = b->num_sources == num_sources;
+    for (int s = 0; same_refs && s < num_sources; ++s)
+        same_refs = b->h_fused[s].t0 == fs[s].t0 && b->h_fused[s].tdi_start == fs[s].tdi_start;
     std::memcpy(b->h_fused, fs.data(), sizeof(FusedSource) * num_sources);
     std::memcpy(b->h_fused_tiles, ft.data(), sizeof(int) * (num_sources + 1));
     FLR_HIP(hipMemcpyAsync(b->d_fused, b->h_fused, sizeof(FusedSource) * num_sources,
@@ -983,7 +990,7 @@ int flr_fused_bind(flr_batch *b, const flr_source *sources, int num_sources, con
     b->num_sources = num_sources;
     b->samples = samples;
     b->fused_tiles = ft[num_sources];
-    b->lti_version = ~0ull;  // new sources: their filters are built now, after the upload
+    if (!same_refs) b->lti_version = ~0ull;  // new sources: their filters are built now
     const int fst = lti_refresh(b, st);
     // recorded after the filters too: flr_fused_create waits on it before returning
     FLR_HIP(hipEventRecord(b->uploaded, st));
