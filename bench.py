@@ -348,10 +348,10 @@ def main() -> int:
     warmed = []
 
     def warm_up(fn):
-        """Bring the GPU to its working clocks (untimed launches) before the first leg."""
-        if warmed:
-            return
-        end = time.perf_counter() + args.warm_seconds
+        """Bring the GPU to its working clocks with untimed calls of the leg about to be timed
+        (--warm-seconds before the first leg, a fifth of that before each later one: the clocks
+        drop while the host prepares a leg)."""
+        end = time.perf_counter() + (args.warm_seconds / 5 if warmed else args.warm_seconds)
         while time.perf_counter() < end:
             for _ in range(20):
                 fn()
@@ -446,6 +446,7 @@ def main() -> int:
             rd.get_projections(h, lam, beta, t0=0.0, t_buffer=T_BUFFER)
             rd.get_tdi_delays()
 
+        warm_up(dropin_step)
         wd, _ = timed(dropin_step, args.steps, args.warmup)
         rd.check_errors()
         line["mixed_c128_dropin"] = {"value": N_SAMPLES * args.steps / wd,
@@ -465,6 +466,7 @@ def main() -> int:
             rf.get_projections(h, lam, beta, t0=0.0, t_buffer=T_BUFFER)
             resf["out"] = rf.get_tdi_delays()
 
+        warm_up(fast_step)
         wf, _ = timed(fast_step, args.steps, args.warmup)
         rf.check_errors()
         out_fast = torch.stack(resf["out"])
@@ -472,6 +474,7 @@ def main() -> int:
                              precision="fp64", exact=False, fused=True)
         pf = bf.plan(h, [N_SAMPLES], [lam], [beta])
         tf = pf.tune(use_cache=False, reps=2)
+        warm_up(pf.run)
         wfu, msfu = timed(pf.run, args.steps, args.warmup)
         pf.check_errors()
         out_fused64 = pf.out_all.view(3, -1)
