@@ -202,9 +202,11 @@ int flr_fused_reserve(flr_response *response, int capacity, double dt, double sa
                       int flags, flr_batch **out);
 /* Bind num_sources <= capacity sources (packed as for flr_fused_create) to a reserved fused
  * batch: validates them on the host, then uploads the source table on `stream` without
- * blocking (from pinned staging; a bind first waits for the previous bind's upload). A launch
- * still running from an earlier bind must be ordered before this upload: bind and run a batch
- * on one stream. */
+ * blocking (from pinned staging; a bind first waits on the host for the previous bind's upload,
+ * and `stream` waits on the device for the batch's last flr_batch_run, whatever stream it ran
+ * on). Runs replayed from a captured graph are not tracked: order those before a bind. The
+ * tiling follows the response's current tables; flr_batch_run re-tiles the bound sources if
+ * the tables change it later (a TDI table with longer delay chains shortens the tiles). */
 int flr_fused_bind(flr_batch *batch, const flr_source *sources, int num_sources,
                    const void *h_all, void *out_all, void *stream);
 /* Enqueue the batch's launches on `stream` (projection then TDI, or the one fused launch). */

@@ -67,6 +67,15 @@ struct DeviceGuard {
 
 const int kStdLinks[kNLinks] = {12, 23, 31, 13, 32, 21};
 
+// Whether `stream` is capturing into a graph (events recorded there are graph-internal).
+int stream_capturing(hipStream_t stream, bool &capturing)
+{
+    hipStreamCaptureStatus status = hipStreamCaptureStatusNone;
+    FLR_HIP(hipStreamIsCapturing(stream, &status));
+    capturing = status != hipStreamCaptureStatusNone;
+    return FLR_OK;
+}
+
 }  // namespace
 
 struct flr_orbits {
@@ -155,6 +164,8 @@ struct flr_batch {
     FusedSource *h_fused = nullptr;  // pinned staging of the source table and tile prefix
     int *h_fused_tiles = nullptr;
     hipEvent_t uploaded = nullptr;   // the last bind's upload
+    hipEvent_t ran = nullptr;        // the last run's launch: a bind on any stream waits for it
+    bool ran_recorded = false;       // `ran` has been recorded at least once
     double *d_lti = nullptr;         // mixed fused: [capacity][kLtiRecord] composite TDI filters
     uint64_t lti_version = ~0ull;    // response version the filters were built for (none yet)
 };
@@ -778,6 +789,10 @@ void flr_batch_destroy(flr_batch *b)
         (void)hipEventSynchronize(b->uploaded);
         (void)hipEventDestroy(b->uploaded);
     }
+    if (b->ran) {
+        if (b->ran_recorded) (void)hipEventSynchronize(b->ran);
+        (void)hipEventDestroy(b->ran);
+    }
     if (b->d_fused) (void)hipFree(b->d_fused);
     if (b->d_fused_tiles) (void)hipFree(b->d_fused_tiles);
     if (b->d_lti) (void)hipFree(b->d_lti);
@@ -894,6 +909,7 @@ int flr_fused_reserve(flr_response *r, int capacity, double dt, double sampling_
     if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_fused, sizeof(FusedSource) * capacity);
     if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_fused_tiles, sizeof(int) * (capacity + 1));
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->uploaded, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ran, hipEventDisableTiming);
     if (e == hipSuccess && (flags & FLR_FLAG_MIXED))
         e = hipMalloc(&b->d_lti, sizeof(double) * kLtiRecord * capacity);
     if (e != hipSuccess) {
@@ -905,6 +921,79 @@ int flr_fused_reserve(flr_response *r, int capacity, double dt, double sampling_
 }
 
 namespace {
+
+// The device record of source `s` of a fused batch (validated): pointers into the packed input
+// (complex128, or 8-byte samples for the complex64 and q31 formats) and the packed float32 or
+// double outputs.
+int fused_source_record(const flr_source &q, int s, int flags, const void *h_all, void *out_all,
+                        FusedSource &f)
+{
+    if (q.num_pts < 1 || q.offset < 0 || q.projections_start_ind < 0 || q.tdi_start_ind < 0)
+        return fail(FLR_ERR_INVALID_ARGUMENT, "source " + std::to_string(s) + ": bad size or offset");
+    if (!std::isfinite(q.scale))
+        return fail(FLR_ERR_INVALID_ARGUMENT, "source " + std::to_string(s) + ": scale is not finite");
+    const size_t hsz = ((flags & FLR_FLAG_MIXED) && !(flags & FLR_FLAG_C128)) ? 2 * sizeof(float)
+                                                                               : 2 * sizeof(double);
+    const size_t osz = (flags & FLR_FLAG_MIXED) ? sizeof(float) : sizeof(double);
+    f = FusedSource{};
+    f.h = static_cast<const char *>(h_all) + q.offset * hsz;
+    f.out = static_cast<char *>(out_all) + 3 * q.offset * osz;
+    f.num_pts = q.num_pts;
+    f.proj_start = q.projections_start_ind;
+    f.tdi_start = q.tdi_start_ind;
+    f.t0 = q.t0;
+    f.scale = q.scale == 0.0 ? 1.0 : q.scale;
+    for (int c = 0; c < 3; ++c) {
+        f.k[c] = q.k[c];
+        f.u[c] = q.u[c];
+        f.v[c] = q.v[c];
+    }
+    return FLR_OK;
+}
+
+// Tile prefix sums of a fused batch's sources for tiles of tile_len output samples: prefix[s]
+// is the first tile of source s, prefix[num_sources] the total.
+int fused_tile_prefix(const FusedSource *src, int num_sources, int zero_edges, int tile_len,
+                      int *prefix)
+{
+    prefix[0] = 0;
+    for (int s = 0; s < num_sources; ++s) {
+        const int64_t nt = (int64_t)prefix[s] + tiles_for(src[s].num_pts, src[s].tdi_start,
+                                                          zero_edges, tile_len);
+        if (nt > INT32_MAX) return fail(FLR_ERR_UNSUPPORTED, "batch has too many tiles");
+        prefix[s + 1] = (int)nt;
+    }
+    return FLR_OK;
+}
+
+// A fused batch's tile length follows its launch configuration, which the response's tables
+// set too (a TDI table with longer delay chains has wider halos and so shorter tiles). When the
+// configuration's tile differs from the one the bound sources were tiled for, re-tile them from
+// the pinned staging copy and upload the prefix on `stream` (after any run still reading it).
+int fused_retile(flr_batch *b, int tile_len, hipStream_t stream)
+{
+    if (tile_len == b->tile_len) return FLR_OK;
+    bool cap = false;
+    int cst = stream_capturing(stream, cap);
+    if (cst) return cst;
+    if (cap)
+        return fail(FLR_ERR_UNSUPPORTED, "the response's tables changed the batch's tiling since "
+                                         "the bind: run it once outside stream capture first");
+    if (b->num_sources > 0) {
+        FLR_HIP(hipEventSynchronize(b->uploaded));  // the staging copy is no longer being read
+        const int zero_edges = (b->flags & FLR_FLAG_ZERO_EDGES) ? 1 : 0;
+        int st = fused_tile_prefix(b->h_fused, b->num_sources, zero_edges, tile_len,
+                                   b->h_fused_tiles);
+        if (st) return st;
+        if (b->ran_recorded) FLR_HIP(hipStreamWaitEvent(stream, b->ran, 0));
+        FLR_HIP(hipMemcpyAsync(b->d_fused_tiles, b->h_fused_tiles,
+                               sizeof(int) * (b->num_sources + 1), hipMemcpyHostToDevice, stream));
+        FLR_HIP(hipEventRecord(b->uploaded, stream));
+        b->fused_tiles = b->h_fused_tiles[b->num_sources];
+    }
+    b->tile_len = tile_len;
+    return FLR_OK;
+}
 
 // Build the bound sources' composite TDI filters (mixed fused batches with equal-arm stock
 // tables) unless they are current: they are set on bind and rebuilt when the response's
@@ -941,39 +1030,25 @@ int flr_fused_bind(flr_batch *b, const flr_source *sources, int num_sources, con
     DeviceGuard g(b->r->device);
     const int flags = b->flags;
     const int zero_edges = (flags & FLR_FLAG_ZERO_EDGES) ? 1 : 0;
-    const size_t hsz = ((flags & FLR_FLAG_MIXED) && !(flags & FLR_FLAG_C128)) ? 2 * sizeof(float)
-                                                                               : 2 * sizeof(double);
-    const size_t osz = (flags & FLR_FLAG_MIXED) ? sizeof(float) : sizeof(double);
+    // the tile the response's current tables give (they may have changed since the reserve)
+    ProjConfig pc;
+    TdiConfig tc;
+    StockTdi stock;
+    int cst = make_fused_config(b->r, b->dt, b->fs, flags, b->tile_passes, pc, tc, stock);
+    if (cst) return cst;
     std::vector<FusedSource> fs(num_sources);
     std::vector<int> ft(num_sources + 1, 0);
     int64_t samples = 0;
     for (int s = 0; s < num_sources; ++s) {
-        const flr_source &q = sources[s];
-        if (q.num_pts < 1 || q.offset < 0 || q.projections_start_ind < 0 || q.tdi_start_ind < 0)
-            return fail(FLR_ERR_INVALID_ARGUMENT, "source " + std::to_string(s) + ": bad size or offset");
-        if (!std::isfinite(q.scale))
-            return fail(FLR_ERR_INVALID_ARGUMENT, "source " + std::to_string(s) + ": scale is not finite");
-        FusedSource &f = fs[s];
-        f.h = static_cast<const char *>(h_all) + q.offset * hsz;
-        f.out = static_cast<char *>(out_all) + 3 * q.offset * osz;
-        f.num_pts = q.num_pts;
-        f.proj_start = q.projections_start_ind;
-        f.tdi_start = q.tdi_start_ind;
-        f.t0 = q.t0;
-        f.scale = q.scale == 0.0 ? 1.0 : q.scale;
-        for (int c = 0; c < 3; ++c) {
-            f.k[c] = q.k[c];
-            f.u[c] = q.u[c];
-            f.v[c] = q.v[c];
-        }
-        const int64_t nt = (int64_t)ft[s] + tiles_for(q.num_pts, f.tdi_start, zero_edges,
-                                                      b->tile_len);
-        if (nt > INT32_MAX) return fail(FLR_ERR_UNSUPPORTED, "batch has too many tiles");
-        ft[s + 1] = (int)nt;
-        samples += q.num_pts;
+        if ((cst = fused_source_record(sources[s], s, flags, h_all, out_all, fs[s]))) return cst;
+        samples += sources[s].num_pts;
     }
+    if ((cst = fused_tile_prefix(fs.data(), num_sources, zero_edges, tc.tile, ft.data()))) return cst;
     hipStream_t st = (hipStream_t)stream;
     FLR_HIP(hipEventSynchronize(b->uploaded));  // the staging buffers are free again
+    // a run still in flight (possibly on another stream) reads the tables about to be replaced
+    if (b->ran_recorded) FLR_HIP(hipStreamWaitEvent(st, b->ran, 0));
+    b->tile_len = tc.tile;
     // A source's composite filters depend on its first time and first TDI sample (and on the
     // response's tables, tracked by lti_version), not on its sky position or input: when these
     // match the previous bind's (still in the staging copy), the filters built then -- already
@@ -1032,9 +1107,11 @@ int flr_batch_run(flr_batch *b, void *stream)
     StockTdi stock;
     int st;
     if (b->fused) {
-        if ((st = lti_refresh(b, s))) return st;
+        if (b->num_sources < 1) return fail(FLR_ERR_INVALID_ARGUMENT, "no sources bound to the batch");
         if ((st = make_fused_config(r, b->dt, b->fs, b->flags, b->tile_passes, pc, tc, stock)))
             return st;
+        if ((st = fused_retile(b, tc.tile, s))) return st;
+        if ((st = lti_refresh(b, s))) return st;
         Batch<FusedSource> bf{b->d_fused, b->d_fused_tiles, b->num_sources, b->fused_tiles};
         tc.lti_fir = b->d_lti;
         hipError_t e = launch_fused_batch(pc, tc, bf, r->orbits->view(), r->lagrange(), exact,
@@ -1044,6 +1121,14 @@ int flr_batch_run(flr_batch *b, void *stream)
                                           b->tile_passes,
                                           b->grid_per_cu, stock, s);
         if (e != hipSuccess) return hip_fail(e, "fused batch launch");
+        // an event recorded inside a capture cannot be waited on outside it: replays of a
+        // captured run are ordered against later binds by the caller
+        bool cap = false;
+        if ((st = stream_capturing(s, cap))) return st;
+        if (!cap) {
+            FLR_HIP(hipEventRecord(b->ran, s));
+            b->ran_recorded = true;
+        }
         return FLR_OK;
     }
     st = make_proj_config(r, b->dt, b->fs, b->flags, pc);
@@ -1087,24 +1172,10 @@ int flr_batch_set_launch(flr_batch *b, int tile_samples, int grid_per_cu)
         StockTdi stock;
         int st = make_fused_config(b->r, b->dt, b->fs, b->flags, passes, pc, tc, stock);
         if (st) return st;
+        // re-tile the bound sources (host copy in the staging buffer), then wait for the upload
+        if ((st = fused_retile(b, tc.tile, nullptr))) return st;
+        FLR_HIP(hipEventSynchronize(b->uploaded));
         b->tile_passes = passes;
-        b->tile_len = tc.tile;
-        if (b->num_sources > 0) {  // re-tile the bound sources (host copy in the staging buffer)
-            FLR_HIP(hipEventSynchronize(b->uploaded));
-            const int zero_edges = (b->flags & FLR_FLAG_ZERO_EDGES) ? 1 : 0;
-            int *ft = b->h_fused_tiles;
-            ft[0] = 0;
-            for (int s = 0; s < b->num_sources; ++s) {
-                const FusedSource &f = b->h_fused[s];
-                const int64_t nt = (int64_t)ft[s] + tiles_for(f.num_pts, f.tdi_start, zero_edges,
-                                                              b->tile_len);
-                if (nt > INT32_MAX) return fail(FLR_ERR_UNSUPPORTED, "batch has too many tiles");
-                ft[s + 1] = (int)nt;
-            }
-            FLR_HIP(hipMemcpy(b->d_fused_tiles, ft, sizeof(int) * (b->num_sources + 1),
-                              hipMemcpyHostToDevice));
-            b->fused_tiles = ft[b->num_sources];
-        }
     }
     if (grid_per_cu != 0) b->grid_per_cu = grid_per_cu;
     return FLR_OK;
@@ -1133,3 +1204,4 @@ int flr_response_set_exact(flr_response *r, int exact)
 }
 
 }  // extern "C"
+

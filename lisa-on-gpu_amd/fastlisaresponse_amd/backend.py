@@ -283,3 +283,14 @@ class LISAResponseWrap:
         if h is not None and h.value and _capi._lib is not None:
             _capi._lib.flr_response_destroy(h)
             self._h = None
+
+
+def backend_methods() -> dict:
+    """The binding classes a backend of the reference package hands to its host code: the
+    fields of FastLISAResponseBackendMethods (src/fastlisaresponse/cutils/__init__.py:13-17)
+    except `xp`. The reference's host layer calls numpy-style methods on its arrays (`.astype`,
+    `.flatten().copy()`, response.py:403-411,607), so a backend built from these classes takes
+    xp=numpy: the classes accept host arrays, stage them to the device for each call and write
+    the outputs back in place."""
+    return dict(LISAResponseWrap=LISAResponseWrap, LISAResponseBase=LISAResponseWrap,
+                OrbitsWrap=OrbitsWrap)

@@ -23,6 +23,7 @@ the float32 outputs and the mixed arithmetic.
 from __future__ import annotations
 
 import ctypes
+import warnings
 from typing import Optional, Sequence
 
 import numpy as np
@@ -35,6 +36,87 @@ from .response import pyResponseTDI
 from .tables import buffer_indices, sky_basis
 
 PRECISIONS = ("fp64", "mixed", "mixed-q31", "mixed-c128")
+
+C_SI = 299792458.0
+
+# ------------------------------------------------------------------ storage-error guard
+# The 8-byte input formats round h to 2^-24 of each value ("mixed", complex64) or 2^-31 of full
+# scale ("mixed-q31"). That rounding is white noise, while the response's signal cancels: the
+# projection differences h at two times ~L apart (a factor ~wL) and TDI differences the links
+# again, so the relative error of A and E grows about as (wL)^-3 towards low frequencies
+# (w = 2 pi f, L = armlength / c), and faster still on the null channel T. The model
+# K (wL)^-3 is calibrated on the 1-yr 5 mHz source (wL = 0.262): complex64 A, E rel-L2 2.5e-7
+# (measured against the FP64 oracle), q31 2.8e-9 (the oracle on the quantised input); it
+# reproduces the 512-source batch's medians (1.2e-6 for f0 in [1, 10] mHz, 9e-4 below).
+STORAGE_K = {"fp64": 0.0, "mixed-c128": 0.0, "mixed": 4.5e-9, "mixed-q31": 5.0e-11}
+STORAGE_TOL = 1e-6
+
+
+class StorageErrorWarning(UserWarning):
+    """An 8-byte input format whose rounding error on the A and E channels, amplified by the
+    response's cancellations, is estimated above the tolerance; precision="mixed-c128" (the
+    complex128 input read directly) has no storage error."""
+
+
+def storage_error_estimate(f, precision: str, armlength: float = 2.5e9):
+    """Estimated rel-L2 error of A and E that the storage format of `precision` causes for a
+    source of (RMS) frequency f Hz: K (2 pi f L / c)^-3; 0 for the formats without storage
+    error, and for f <= 0 (no signal)."""
+    k = STORAGE_K[precision]
+    f = np.asarray(f, dtype=np.float64)
+    wl = 2.0 * np.pi * f * armlength / C_SI
+    with np.errstate(divide="ignore", invalid="ignore"):
+        est = np.where(wl > 0.0, k / np.maximum(wl, 1e-300) ** 3, 0.0)
+    return est if est.ndim else float(est)
+
+
+def rms_frequencies(h: torch.Tensor, lengths: Sequence[int], dt: float) -> np.ndarray:
+    """RMS frequency (Hz) of each source of a packed complex series: with r^2 = mean |h[i+1] -
+    h[i]|^2 / mean |h[i]|^2 over the source, f = 2 arcsin(r / 2) / (2 pi dt), which is exact for a
+    single complex sinusoid. One pass over the samples (per-source sums by index), one host
+    sync."""
+    lengths = np.asarray([int(n) for n in lengths], dtype=np.int64)
+    if h.dtype == torch.int32:  # q31 pairs
+        h = torch.view_as_complex(h.view(-1, 2).to(torch.float64).contiguous())
+    h = h.to(torch.complex128) if h.dtype != torch.complex128 else h
+    nsrc = len(lengths)
+    ids = torch.repeat_interleave(torch.arange(nsrc, device=h.device),
+                                  torch.as_tensor(lengths, device=h.device))
+    p = torch.zeros(nsrc, dtype=torch.float64, device=h.device)
+    d = torch.zeros(nsrc, dtype=torch.float64, device=h.device)
+    p.index_add_(0, ids, h.abs().square())
+    if h.numel() > 1:
+        diff = (h[1:] - h[:-1]).abs().square()
+        same = ids[1:] == ids[:-1]  # no differences across source boundaries
+        d.index_add_(0, ids[1:][same], diff[same])
+    p, d = p.cpu().numpy() / np.maximum(lengths, 1), d.cpu().numpy() / np.maximum(lengths - 1, 1)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        r = np.sqrt(np.where(p > 0.0, d / np.maximum(p, 1e-300), 0.0))
+    return 2.0 * np.arcsin(np.minimum(r, 2.0) / 2.0) / (2.0 * np.pi * dt)
+
+
+def check_storage(h: torch.Tensor, lengths: Sequence[int], dt: float, precision: str,
+                  armlength: float = 2.5e9, tol: float = STORAGE_TOL, action: str = "warn"):
+    """Estimate each source's storage error (storage_error_estimate at its RMS frequency) and
+    warn (StorageErrorWarning) or raise (ValueError, action="raise") when one exceeds tol;
+    returns the estimates. action="off" or a format without storage error skips the pass."""
+    if action not in ("warn", "raise", "off"):
+        raise ValueError("action must be 'warn', 'raise' or 'off'")
+    if action == "off" or STORAGE_K[precision] == 0.0:
+        return np.zeros(len(lengths))
+    est = np.atleast_1d(storage_error_estimate(rms_frequencies(h, lengths, dt), precision,
+                                               armlength))
+    bad = np.flatnonzero(est > tol)
+    if bad.size:
+        msg = (f"precision={precision!r}: {bad.size} of {len(est)} source(s) have an estimated "
+               f"storage error on A and E up to {est.max():.1e} (rel-L2, > {tol:g}; the null "
+               "channel T is worse) -- the input rounding is amplified ~(2 pi f L/c)^-3 at low "
+               "frequencies; precision='mixed-c128' reads the complex128 input with no storage "
+               "error at the same speed")
+        if action == "raise":
+            raise ValueError(msg)
+        warnings.warn(msg, StorageErrorWarning, stacklevel=3)
+    return est
 
 
 def pack_mixed(h: torch.Tensor, lengths: Sequence[int]):
@@ -90,11 +172,14 @@ class BatchedResponse:
     on chip, order 25) and precision ("fp64", "mixed" with complex64 input, "mixed-q31"
     with 32-bit fixed-point input, see pack_q31, or "mixed-c128" with complex128 input; the
     mixed forms imply fused and the fast stencils and write float32 outputs). exact defaults to
-    True for "fp64" and False otherwise."""
+    True for "fp64" and False otherwise. storage_check ("warn", "raise" or "off"): for "mixed"
+    and "mixed-q31", each plan estimates every source's storage error on A and E
+    (check_storage) and warns (StorageErrorWarning) or raises above 1e-6."""
 
     def __init__(self, sampling_frequency: float, order: int = 25, tdi="2nd generation",
                  orbits=None, tdi_chan: str = "AET", exact: Optional[bool] = None,
-                 t_buffer: float = 10000.0, fused: bool = False, precision: str = "fp64"):
+                 t_buffer: float = 10000.0, fused: bool = False, precision: str = "fp64",
+                 storage_check: str = "warn"):
         if tdi_chan not in ("XYZ", "AET", "AE"):
             raise ValueError("tdi_chan must be 'XYZ', 'AET' or 'AE'.")
         if precision not in PRECISIONS:
@@ -116,6 +201,9 @@ class BatchedResponse:
         self.exact = bool(exact)
         self.fused = bool(fused)
         self.precision = precision
+        if storage_check not in ("warn", "raise", "off"):
+            raise ValueError("storage_check must be 'warn', 'raise' or 'off'")
+        self.storage_check = storage_check
         self.t_buffer = t_buffer
         (self.tdi_start_ind, _, self.projection_buffer,
          self.projections_start_ind) = buffer_indices(sampling_frequency, order, t_buffer,
@@ -159,23 +247,14 @@ class BatchPlan:
         self.br = br
         self.h_all = h_all
         self.lengths = lengths
+        self.storage_error = check_storage(h_all, lengths, br.dt, br.precision,
+                                           br.model.response_orbits.armlength, action=br.storage_check)
         self.offsets = np.concatenate([[0], np.cumsum(lengths)[:-1]]).astype(np.int64)
         dev = h_all.device
         self.y_all = None if br.fused else torch.empty(6 * total, dtype=torch.float64, device=dev)
         self.out_all = torch.empty(3 * total, dtype=torch.float32 if mixed else torch.float64,
                                    device=dev)
-        srcs = (_capi.FlrSource * n)()
-        for s in range(n):
-            k, u, v = sky_basis(lams[s], betas[s])
-            q = srcs[s]
-            q.offset = int(self.offsets[s])
-            q.num_pts = int(lengths[s])
-            q.projections_start_ind = br.projections_start_ind
-            q.tdi_start_ind = br.tdi_start_ind
-            q.t0 = 0.0 if t0s is None else float(t0s[s])
-            q.scale = 1.0 if scales is None else float(scales[s])
-            for c in range(3):
-                q.k[c], q.u[c], q.v[c] = k[c], u[c], v[c]
+        srcs = self._records(lams, betas, t0s, scales)
         flags = _capi.FLR_FLAG_ZERO_EDGES | _capi.FLR_FLAG_OVERWRITE
         if br.tdi_chan != "XYZ":
             flags |= _capi.FLR_FLAG_AET
@@ -200,6 +279,51 @@ class BatchPlan:
         self.num_sources = n
         self.samples = int(lib().flr_batch_samples(h))
         self.launch_shape = (512 if br.fused else 256, 32)  # (tile window samples, WGs per CU)
+
+    def _records(self, lams, betas, t0s=None, scales=None):
+        """flr_source records of the plan's sources (packed offsets, the lengths of the plan)."""
+        n = len(self.lengths)
+        if (len(lams) != n or len(betas) != n or (t0s is not None and len(t0s) != n)
+                or (scales is not None and len(scales) != n)):
+            raise ValueError("lams, betas (and t0s, scales) must have one entry per source")
+        srcs = (_capi.FlrSource * n)()
+        for s in range(n):
+            k, u, v = sky_basis(lams[s], betas[s])
+            q = srcs[s]
+            q.offset = int(self.offsets[s])
+            q.num_pts = int(self.lengths[s])
+            q.projections_start_ind = self.br.projections_start_ind
+            q.tdi_start_ind = self.br.tdi_start_ind
+            q.t0 = 0.0 if t0s is None else float(t0s[s])
+            q.scale = 1.0 if scales is None else float(scales[s])
+            for c in range(3):
+                q.k[c], q.u[c], q.v[c] = k[c], u[c], v[c]
+        return srcs
+
+    def rebind(self, lams, betas, t0s=None, scales=None, h_all: Optional[torch.Tensor] = None,
+               stream: Optional[int] = None) -> None:
+        """Fused plans: bind new source parameters (sky positions, start times, scales) and
+        optionally a new packed input of the same format and size, keeping the lengths, the
+        outputs and the launch shape -- the MCMC pattern of many steps over fixed-length
+        sources. Enqueued on `stream` (default: the current one) without allocating; the
+        composite TDI filters are rebuilt only if a start time changed. A graph captured from
+        run() keeps replaying correctly after a rebind, since the kernel reads the rebound
+        tables at run time."""
+        if not self.br.fused:
+            raise ValueError("rebind needs a fused plan (fused=True or a mixed precision)")
+        if h_all is not None:
+            if (not isinstance(h_all, torch.Tensor) or h_all.dtype != self.h_all.dtype or
+                    h_all.numel() != self.h_all.numel() or not h_all.is_cuda or
+                    not h_all.is_contiguous()):
+                raise TypeError("h_all must be a contiguous device tensor like the plan's input")
+            self.storage_error = check_storage(h_all, self.lengths, self.br.dt, self.br.precision,
+                                               self.br.model.response_orbits.armlength,
+                                               action=self.br.storage_check)
+            self.h_all = h_all
+        srcs = self._records(lams, betas, t0s, scales)
+        check(lib().flr_fused_bind(self._h, srcs, len(self.lengths), self.h_all.data_ptr(),
+                                   self.out_all.data_ptr(),
+                                   current_stream() if stream is None else stream))
 
     def tune(self, **kw) -> dict:
         """Pick the fastest launch shape for this plan (fastlisaresponse_amd.autotune.tune)."""

@@ -113,12 +113,16 @@ class pyResponseTDI:
         precision (str): "fp64" (default), "mixed" (complex64 input, float32 outputs, fused
             launch, order 25), "mixed-q31" (the same with 32-bit fixed-point input) or
             "mixed-c128" (the same reading the complex128 input).
+        storage_check (str): "warn" (default), "raise" or "off": with "mixed" and "mixed-q31",
+            get_projections estimates the input's storage error on A and E from its RMS
+            frequency (batch.check_storage) and warns (StorageErrorWarning) or raises when it
+            exceeds 1e-6.
     """
 
     def __init__(self, sampling_frequency, num_pts, order=25, tdi="1st generation",
                  orbits: Optional[Orbits] = None, tdi_orbits: Optional[Orbits] = None,
                  tdi_chan="XYZ", force_backend=None, exact: Optional[bool] = None,
-                 precision: str = "fp64"):
+                 precision: str = "fp64", storage_check: str = "warn"):
         _check_backend(force_backend)
         if precision not in ("fp64", "mixed", "mixed-q31", "mixed-c128"):
             raise ValueError("precision must be 'fp64', 'mixed', 'mixed-q31' or 'mixed-c128'")
@@ -129,6 +133,9 @@ class pyResponseTDI:
         if precision != "fp64" and order != 25:
             raise ValueError(f"precision={precision!r} is built for order 25")
         self.precision = precision
+        if storage_check not in ("warn", "raise", "off"):
+            raise ValueError("storage_check must be 'warn', 'raise' or 'off'")
+        self.storage_check = storage_check
         self.sampling_frequency = sampling_frequency
         self.dt = 1 / sampling_frequency
         self.tdi_buffer = 200
@@ -328,10 +335,12 @@ class pyResponseTDI:
         h = h.to(device=self.device, dtype=torch.complex128).contiguous()
         self._input_keepalive = h
         if self.precision != "fp64":
-            from .batch import pack_mixed, pack_q31
+            from .batch import check_storage, pack_mixed, pack_q31
             if self.precision == "mixed-c128":
                 stored, scale = h, 1.0
             else:
+                check_storage(h, [len(h)], self.dt, self.precision,
+                              self.response_orbits.armlength, action=self.storage_check)
                 pack = pack_q31 if self.precision == "mixed-q31" else pack_mixed
                 stored, scales = pack(h, [len(h)])
                 scale = scales[0]

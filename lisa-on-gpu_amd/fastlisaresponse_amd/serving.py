@@ -7,7 +7,10 @@ handle (orbit, Lagrange and TDI tables on the device) and, per stream, one reser
 (device source table + pinned staging), so the loop allocates no device tables and never
 synchronises the device:
 
-* arrival   -- `submit` queues the request (FIFO).
+* arrival   -- `submit` queues the request (FIFO). Inputs the client passes as device tensors
+  may still be in production on the client's current stream: submit records an event there,
+  the launch's stream waits for it before copying, and the copy keeps the tensor's memory from
+  being reused until it has been read, so the client needs no synchronisation.
 * admission -- the head request is admitted when the bytes in flight plus its own (packed input
   and float32 outputs) fit the HBM budget and a stream is free; queued requests with the same
   key are merged into the same launch up to `max_launch_samples` / `max_sources`.
@@ -27,6 +30,7 @@ from __future__ import annotations
 import ctypes
 import os
 import time
+import warnings
 from collections import deque
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Sequence, Tuple
@@ -36,7 +40,8 @@ import torch
 
 from . import _capi
 from ._capi import check, lib
-from .batch import BatchedResponse
+from .batch import (C_SI, STORAGE_K, STORAGE_TOL, BatchedResponse, StorageErrorWarning,
+                    storage_error_estimate)
 from .tables import sky_basis
 
 
@@ -70,6 +75,8 @@ class Request:
     completed: float = 0.0
     result: Optional[torch.Tensor] = None  # [3 * samples] float32 (A, E, T per source) if kept
     offsets: List[int] = field(default_factory=list)
+    keep: Optional[bool] = None  # keep this request's result (None: the server's keep_results)
+    ready: Optional[torch.cuda.Event] = None  # recorded at submit on the caller's stream
 
     @property
     def key(self) -> Tuple[float, str]:
@@ -106,10 +113,13 @@ class ResponseServer:
     Args:
         orbits: orbit provider shared by all requests (tables must cover the longest request).
         keys: (dt, tdi) pairs to keep resident.
-        hbm_budget: bytes of request inputs + outputs allowed in flight.
+        hbm_budget: bytes of request inputs + outputs allowed in flight; None (default) is
+            90% of the device's memory (259 GB of an MI355X's 288 GB), the rest left to the
+            resident tables, torch's allocator cache and other users.
         num_streams: launches that may run concurrently.
         max_launch_samples, max_sources: caps on one merged launch.
-        keep_results: keep each request's outputs (tests); otherwise they are dropped.
+        keep_results: keep each request's outputs (tests); otherwise they are dropped. A
+            request's own `keep` (True or False) overrides it.
         t_buffer: edge length (s) left out of TDI, as pyResponseTDI.get_projections.
         precision: "mixed-c128" (complex128 inputs, float32 outputs; default) or "mixed"
             (complex64 inputs h / scale: half the input memory, FP32 storage error).
@@ -121,7 +131,7 @@ class ResponseServer:
     INPUT_BYTES = {"mixed": 8, "mixed-c128": 16}
     BYTES_PER_SAMPLE = 16 + 12  # per sample in flight: input + float32 A, E, T (default format)
 
-    def __init__(self, orbits, keys: Sequence[Tuple[float, str]], hbm_budget: int = 64 << 30,
+    def __init__(self, orbits, keys: Sequence[Tuple[float, str]], hbm_budget: Optional[int] = None,
                  num_streams: int = 2, max_launch_samples: int = 1 << 27, max_sources: int = 512,
                  keep_results: bool = False, t_buffer: float = 10000.0, tdi_chan: str = "AET",
                  precision: str = "mixed-c128", launch: Optional[Tuple[int, int]] = (1024, 32)):
@@ -133,6 +143,15 @@ class ResponseServer:
             # streams on 4 queues served a quarter of the throughput)
             raise ValueError(f"num_streams must be in [1, {queues}] (hardware queues per process)")
         self.precision = precision
+        if precision == "mixed":
+            # the clients' complex64 inputs are not inspected in the loop (that would synchronise
+            # per request); parameter requests are checked at submit from their frequency
+            f_lim = (STORAGE_K[precision] / STORAGE_TOL) ** (1.0 / 3.0) * C_SI / (
+                2.0 * np.pi * orbits.armlength)
+            warnings.warn(f"precision='mixed': the complex64 storage error on A and E exceeds "
+                          f"{STORAGE_TOL:g} (rel-L2) for sources below ~{f_lim * 1e3:.1f} mHz; "
+                          "'mixed-c128' has none", StorageErrorWarning, stacklevel=2)
+        self.armlength = float(orbits.armlength)
         self.launch = None if launch is None else (int(launch[0]), int(launch[1]))
         self.BYTES_PER_SAMPLE = self.INPUT_BYTES[precision] + 12
         self._in_dtype = torch.complex128 if precision == "mixed-c128" else torch.complex64
@@ -141,6 +160,9 @@ class ResponseServer:
             self.models[(dt, tdi)] = BatchedResponse(1.0 / dt, tdi=tdi, orbits=orbits,
                                                      tdi_chan=tdi_chan, precision=precision,
                                                      t_buffer=t_buffer)
+        if hbm_budget is None:
+            hbm_budget = int(0.9 * torch.cuda.get_device_properties(torch.cuda.current_device())
+                             .total_memory)
         self.hbm_budget = int(hbm_budget)
         self.max_launch_samples = int(max_launch_samples)
         self.max_sources = int(max_sources)
@@ -197,6 +219,18 @@ class ResponseServer:
         if need > self.hbm_budget:
             raise ValueError("request larger than the HBM budget")
         req.arrival = time.perf_counter() if now is None else now
+        if self.precision == "mixed":
+            f0 = [sp.gb[1] for sp in req.sources if sp.h is None]
+            if f0 and max(np.atleast_1d(storage_error_estimate(f0, self.precision,
+                                                                self.armlength))) > STORAGE_TOL:
+                warnings.warn(f"request {req.rid}: a source's estimated complex64 storage error "
+                              f"on A and E exceeds {STORAGE_TOL:g}; use precision='mixed-c128'",
+                              StorageErrorWarning, stacklevel=2)
+        if any(sp.h is not None for sp in req.sources):
+            # the inputs may still be being produced on the caller's stream: the launch that
+            # copies them waits for this point of it (no host synchronisation)
+            req.ready = torch.cuda.Event()
+            req.ready.record(torch.cuda.current_stream())
         self.pending.append(req)
 
     def poll(self) -> int:
@@ -208,7 +242,7 @@ class ResponseServer:
             now = time.perf_counter()
             for req in ln.requests:
                 req.completed = now
-                if not self.keep_results:
+                if not (self.keep_results if req.keep is None else req.keep):
                     req.result = None
                 self.done.append(req)
                 n += 1
@@ -277,11 +311,16 @@ class ResponseServer:
             for req in reqs:
                 req.admitted = now
                 req.offsets = []
+                if req.ready is not None:
+                    stream.wait_event(req.ready)
                 for sp in req.sources:
                     n = sp.num_pts
                     s = srcs[q]
                     if sp.h is not None:
                         arena[o:o + n].copy_(sp.h, non_blocking=True)
+                        # the caller may drop its tensor now: its memory is not reused before
+                        # this stream's copy has read it
+                        sp.h.record_stream(stream)
                         s.scale = sp.scale
                     else:  # generated below, all of the launch's parameter sources at once
                         gen_params.append(sp.gb)

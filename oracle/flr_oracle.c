@@ -24,12 +24,25 @@
  *
  * Build with -ffp-contract=off: the reference's operation sequence has no fused multiply-adds
  * and the time arithmetic must round exactly as written.
+ *
+ * Wide build (-DORACLE_WIDE, liboracle_flr_wide.so): the value arithmetic -- stencil sums and
+ * prefactors, link geometry, projections, TDI sums, AET -- runs in x87 extended precision
+ * (long double, 64-bit significand), while the time, delay and index arithmetic stays the FP64
+ * sequence above, so every stencil sits at the same sample position with the same fraction as
+ * in the FP64 build. Its results approximate the FP64 path's exact values with ~2^11 less
+ * rounding: the reference against which the FP64 path's own rounding floor is measured.
  */
 #include <math.h>
 #include <stdint.h>
 #include <string.h>
 #ifdef _OPENMP
 #include <omp.h>
+#endif
+
+#ifdef ORACLE_WIDE
+typedef long double real_t; /* value arithmetic, extended precision */
+#else
+typedef double real_t;      /* value arithmetic, FP64 as the reference */
 #endif
 
 #define ORACLE_NLINKS 6
@@ -103,66 +116,66 @@ void oracle_pos(const oracle_orbits *o, double t, int sc, double out[3])
 /* ---------------------------------------------------------------- stencils */
 
 /* Real order-(2h-1) Lagrange fractional-delay stencil, LISAResponse.cu:110-147. */
-static double interp_single(const double *input, int h, int d, double e, const double *A_arr,
+static real_t interp_single(const real_t *input, int h, int d, double e, const double *A_arr,
                             double deps, const double *E_arr)
 {
 #if ORACLE_MUTANT == 6
     d += 1;
 #endif
     int ind = (int)(e / deps);
-    double frac = (e - ind * deps) / deps;
-    double A = A_arr[ind] * (1. - frac) + A_arr[ind + 1] * frac;
-    double B = 1.0 - e;
-    double C = e;
-    double D = e * (1.0 - e);
-    double sum = 0.0;
+    real_t frac = ((real_t)e - ind * (real_t)deps) / (real_t)deps;
+    real_t A = A_arr[ind] * (1. - frac) + A_arr[ind + 1] * frac;
+    real_t B = 1.0 - (real_t)e;
+    real_t C = e;
+    real_t D = (real_t)e * (1.0 - (real_t)e);
+    real_t sum = 0.0;
     for (int j = 1; j < h; j += 1) {
-        double E = E_arr[j - 1];
-        double F = j + e;
-        double G = j + (1 - e);
-        double up = input[d + 1 + j];
-        double down = input[d - j];
+        real_t E = E_arr[j - 1];
+        real_t F = j + (real_t)e;
+        real_t G = j + (1 - (real_t)e);
+        real_t up = input[d + 1 + j];
+        real_t down = input[d - j];
         sum += E * (up / F + down / G);
     }
-    double up = input[d + 1];
-    double down = input[d];
+    real_t up = input[d + 1];
+    real_t down = input[d];
     return A * (B * up + C * down + D * sum);
 }
 
 /* Complex stencil on interleaved (re, im) doubles, LISAResponse.cu:149-207. */
-static void interp_cplx(double *out_re, double *out_im, const double *input, int h, int d, double e,
+static void interp_cplx(real_t *out_re, real_t *out_im, const double *input, int h, int d, double e,
                         const double *A_arr, double deps, const double *E_arr)
 {
     int ind = (int)(e / deps);
-    double frac = (e - ind * deps) / deps;
-    double A = A_arr[ind] * (1. - frac) + A_arr[ind + 1] * frac;
-    double B = 1.0 - e;
-    double C = e;
-    double D = e * (1.0 - e);
-    double sum_re = 0.0, sum_im = 0.0;
+    real_t frac = ((real_t)e - ind * (real_t)deps) / (real_t)deps;
+    real_t A = A_arr[ind] * (1. - frac) + A_arr[ind + 1] * frac;
+    real_t B = 1.0 - (real_t)e;
+    real_t C = e;
+    real_t D = (real_t)e * (1.0 - (real_t)e);
+    real_t sum_re = 0.0, sum_im = 0.0;
     for (int j = 1; j < h; j += 1) {
-        double E = E_arr[j - 1];
-        double F = j + e;
-        double G = j + (1 - e);
+        real_t E = E_arr[j - 1];
+        real_t F = j + (real_t)e;
+        real_t G = j + (1 - (real_t)e);
         const double *up = &input[2 * (d + 1 + j)];
         const double *down = &input[2 * (d - j)];
-        sum_re += E * (up[0] / F + down[0] / G);
-        sum_im += E * (up[1] / F + down[1] / G);
+        sum_re += E * ((real_t)up[0] / F + (real_t)down[0] / G);
+        sum_im += E * ((real_t)up[1] / F + (real_t)down[1] / G);
     }
     const double *up = &input[2 * (d + 1)];
     const double *down = &input[2 * d];
-    *out_re = A * (B * up[0] + C * down[0] + D * sum_re);
-    *out_im = A * (B * up[1] + C * down[1] + D * sum_im);
+    *out_re = A * (B * (real_t)up[0] + C * (real_t)down[0] + D * sum_re);
+    *out_im = A * (B * (real_t)up[1] + C * (real_t)down[1] + D * sum_im);
 }
 
 /* The standalone stencils, exposed for the known-answer tests. */
-double oracle_interp_single(const double *input, int h, int d, double e, const double *A_arr,
+real_t oracle_interp_single(const real_t *input, int h, int d, double e, const double *A_arr,
                             double deps, const double *E_arr)
 {
     return interp_single(input, h, d, e, A_arr, deps, E_arr);
 }
 
-void oracle_interp_cplx(double *out2, const double *input, int h, int d, double e, const double *A_arr,
+void oracle_interp_cplx(real_t *out2, const double *input, int h, int d, double e, const double *A_arr,
                         double deps, const double *E_arr)
 {
     interp_cplx(&out2[0], &out2[1], input, h, d, e, A_arr, deps, E_arr);
@@ -189,6 +202,14 @@ static double dot3(const double *a, const double *b)
     return out;
 }
 
+/* The same dot product in the value arithmetic's type (the link geometry). */
+static real_t dot3_r(const real_t *a, const double *b)
+{
+    real_t out = 0.0;
+    for (int i = 0; i < 3; i++) out += a[i] * (real_t)b[i];
+    return out;
+}
+
 /* ---------------------------------------------------------------- projection */
 
 /* LISAResponse::get_response CPU path, LISAResponse.cu:431-672 and :674-718.
@@ -196,7 +217,7 @@ static double dot3(const double *a, const double *b)
  * samples in [projections_start_ind, num_delays - projections_start_ind) are written.
  * nthreads > 1 splits the sample loop; every output element is still produced by exactly
  * the same operation sequence, so the result does not depend on nthreads. */
-int oracle_get_response(double *y_gw, const double *t_data, const double *k, const double *u,
+int oracle_get_response(real_t *y_gw, const double *t_data, const double *k, const double *u,
                         const double *v, double dt, int num_delays, const double *input_in,
                         int num_inputs, int order, double sampling_frequency, int buffer_integer,
                         const double *A_in, double deps, int num_A, const double *E_in,
@@ -218,7 +239,8 @@ int oracle_get_response(double *y_gw, const double *t_data, const double *k, con
 #pragma omp parallel for num_threads(nthreads) schedule(static)
 #endif
         for (int i = P; i < num_delays - P; i++) {
-            double x_rec[3], x_em[3], n[3];
+            double x_rec[3], x_em[3];
+            real_t n[3];
             double t = t_data[i];
             double L = oracle_ltt(orbits, t, link);
             double t_rec = t;
@@ -230,24 +252,28 @@ int oracle_get_response(double *y_gw, const double *t_data, const double *k, con
             oracle_pos(orbits, t_rec, sc_r, x_rec);
             oracle_pos(orbits, t_em, sc_e, x_em);
 #endif
-            double norm = 0.0;
+            real_t norm = 0.0;
             for (int c = 0; c < 3; c++) {
-                double nt = x_rec[c] - x_em[c];
+                real_t nt = (real_t)x_rec[c] - (real_t)x_em[c];
                 n[c] = nt;
                 norm += nt * nt;
             }
+#ifdef ORACLE_WIDE
+            norm = sqrtl(norm);
+#else
             norm = sqrt(norm);
+#endif
             for (int c = 0; c < 3; c++) n[c] = n[c] / norm;
 
-            double u_dot_n = dot3(u, n);
-            double v_dot_n = dot3(v, n);
-            double xi_p = 0.5 * ((u_dot_n * u_dot_n) - (v_dot_n * v_dot_n));
-            double xi_c = u_dot_n * v_dot_n;
+            real_t u_dot_n = dot3_r(n, u);
+            real_t v_dot_n = dot3_r(n, v);
+            real_t xi_p = 0.5 * ((u_dot_n * u_dot_n) - (v_dot_n * v_dot_n));
+            real_t xi_c = u_dot_n * v_dot_n;
 #if ORACLE_MUTANT == 3
             xi_c = -xi_c;
 #endif
 
-            double k_dot_n = dot3(k, n);
+            real_t k_dot_n = dot3_r(n, k);
             double k_dot_x_rec = dot3(k, x_rec);
             double k_dot_x_em = dot3(k, x_em);
 
@@ -263,12 +289,12 @@ int oracle_get_response(double *y_gw, const double *t_data, const double *k, con
             split_delay(delay_rec, t0_offset, sampling_frequency, &d_rec, &e_rec);
             split_delay(delay_em, t0_offset, sampling_frequency, &d_em, &e_em);
 
-            double hp_rec, hc_rec, hp_em, hc_em;
+            real_t hp_rec, hc_rec, hp_em, hc_em;
             interp_cplx(&hp_rec, &hc_rec, input_in, h, d_rec, e_rec, A_in, deps, E_in);
             interp_cplx(&hp_em, &hc_em, input_in, h, d_em, e_em, A_in, deps, E_in);
 
-            double pre_factor = 1. / (1. - k_dot_n);
-            double large_factor = (hp_em - hp_rec) * xi_p + (hc_em - hc_rec) * xi_c;
+            real_t pre_factor = 1. / (1. - k_dot_n);
+            real_t large_factor = (hp_em - hp_rec) * xi_p + (hc_em - hc_rec) * xi_c;
             y_gw[link_i * num_delays + i] = pre_factor * large_factor;
         }
     }
@@ -280,7 +306,7 @@ int oracle_get_response(double *y_gw, const double *t_data, const double *k, con
 /* LISAResponse::get_tdi_delays CPU path, LISAResponse.cu:218-397 and :399-429.
  * delayed_links [num_channels][num_delays] is accumulated into (the caller zeroes it),
  * unit by unit in index order, exactly as the reference's `+=` does. */
-int oracle_get_tdi_delays(double *delayed_links, const double *input_links, int num_inputs,
+int oracle_get_tdi_delays(real_t *delayed_links, const real_t *input_links, int num_inputs,
                           int num_delays, const double *t_arr, const int *unit_starts,
                           const int *unit_lengths, const int *tdi_base_link,
                           const int *tdi_link_combinations, const double *tdi_signs_in,
@@ -326,7 +352,7 @@ int oracle_get_tdi_delays(double *delayed_links, const double *input_links, int 
             int d;
             double e;
             split_delay(delay, t0_offset, sampling_frequency, &d, &e);
-            double val = interp_single(&input_links[(int64_t)base_row * num_inputs], h, d, e, A_in,
+            real_t val = interp_single(&input_links[(int64_t)base_row * num_inputs], h, d, e, A_in,
                                        deps, E_in);
             val *= sign;
             delayed_links[(int64_t)channel * num_delays + i] += val;
@@ -339,12 +365,16 @@ int oracle_get_tdi_delays(double *delayed_links, const double *input_links, int 
 
 /* XYZ -> AET as lisatools.utils.utility.AET computes it (call site response.py:665):
  * A = (Z - X)/sqrt(2), E = (X - 2Y + Z)/sqrt(6), T = (X + Y + Z)/sqrt(3). */
-void oracle_aet(const double *X, const double *Y, const double *Z, double *A, double *E, double *T,
+void oracle_aet(const real_t *X, const real_t *Y, const real_t *Z, real_t *A, real_t *E, real_t *T,
                 int64_t n)
 {
+#ifdef ORACLE_WIDE
+    const real_t s2 = sqrtl(2.0L), s6 = sqrtl(6.0L), s3 = sqrtl(3.0L);
+#else
     const double s2 = sqrt(2.0), s6 = sqrt(6.0), s3 = sqrt(3.0);
+#endif
     for (int64_t i = 0; i < n; i++) {
-        double x = X[i], y = Y[i], z = Z[i];
+        real_t x = X[i], y = Y[i], z = Z[i];
 #if ORACLE_MUTANT == 5
         A[i] = (x - z) / s2;
 #else
@@ -353,6 +383,16 @@ void oracle_aet(const double *X, const double *Y, const double *Z, double *A, do
         E[i] = (x - 2.0 * y + z) / s6;
         T[i] = (x + y + z) / s3;
     }
+}
+
+/* 1 in the wide build (liboracle_flr_wide.so), 0 in the FP64 one. */
+int oracle_is_wide(void)
+{
+#ifdef ORACLE_WIDE
+    return 1;
+#else
+    return 0;
+#endif
 }
 
 int oracle_max_threads(void)

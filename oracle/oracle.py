@@ -30,6 +30,9 @@ import numpy as np
 
 ORACLE_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(ORACLE_DIR, "liboracle_flr.so")
+# The same restatement with its value arithmetic in x87 extended precision (time, delay and
+# index arithmetic unchanged): the FP64 path's exact values to ~2^-64 instead of ~2^-53.
+WIDE_LIB_PATH = os.path.join(ORACLE_DIR, "liboracle_flr_wide.so")
 C_INV = 3.3356409519815204e-09  # LISAResponse.hh:7, response.py:36
 
 # The stock TDI combinations, response.py:318-356 (1st generation = first 8 entries).
@@ -75,6 +78,7 @@ class _Orbits(ctypes.Structure):
 
 
 _lib = None
+_wide_lib = None
 
 
 def build_mutant(mutant: int, out_dir: str) -> str:
@@ -87,13 +91,26 @@ def build_mutant(mutant: int, out_dir: str) -> str:
     return out
 
 
-def lib():
-    global _lib
+def lib(wide: bool = False):
+    """The FP64 oracle library, or (wide=True) its extended-precision twin."""
+    global _lib, _wide_lib
+    if wide:
+        if _wide_lib is None:
+            if not os.path.exists(WIDE_LIB_PATH):
+                build()
+            _wide_lib = load(WIDE_LIB_PATH)
+            assert _wide_lib.oracle_is_wide() == 1
+        return _wide_lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
         _lib = load(LIB_PATH)
     return _lib
+
+
+def _real(wide: bool):
+    """numpy type of the value arithmetic: float64, or the C long double of the wide build."""
+    return np.longdouble if wide else np.float64
 
 
 def load(path: str):
@@ -120,6 +137,7 @@ def load(path: str):
     L.oracle_aet.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.c_int64]
     L.oracle_aet.restype = None
     L.oracle_max_threads.restype = ci
+    L.oracle_is_wide.restype = ci
     return L
 
 
@@ -301,16 +319,18 @@ def split_delay(delay: float, t0: float, fs: float):
 
 
 def get_response(orbits: OracleOrbits, input_in: np.ndarray, t_data: np.ndarray, k, u, v, order,
-                 sampling_frequency, A_in, deps, E_in, projections_start_ind, nthreads=1):
-    """y_gw [6][N] (zeros outside the computed range), like get_projections + get_response."""
+                 sampling_frequency, A_in, deps, E_in, projections_start_ind, nthreads=1,
+                 wide=False):
+    """y_gw [6][N] (zeros outside the computed range), like get_projections + get_response;
+    wide=True: the extended-precision build, y as long double."""
     h = np.ascontiguousarray(input_in, dtype=np.complex128)
     t_data = np.ascontiguousarray(t_data, dtype=np.float64)
     n = len(h)
-    y = np.zeros(6 * n)
+    y = np.zeros(6 * n, dtype=_real(wide))
     k, u, v = (np.ascontiguousarray(a, dtype=np.float64) for a in (k, u, v))
     A_in = np.ascontiguousarray(A_in, dtype=np.float64)
     E_in = np.ascontiguousarray(E_in, dtype=np.float64)
-    st = lib().oracle_get_response(_p(y), _p(t_data), _p(k), _p(u), _p(v), 1.0 / sampling_frequency,
+    st = lib(wide).oracle_get_response(_p(y), _p(t_data), _p(k), _p(u), _p(v), 1.0 / sampling_frequency,
                                    n, _p(h), n, order, sampling_frequency, 2 * order + 1, _p(A_in),
                                    deps, len(A_in), _p(E_in), projections_start_ind, orbits.ptr,
                                    nthreads)
@@ -320,16 +340,19 @@ def get_response(orbits: OracleOrbits, input_in: np.ndarray, t_data: np.ndarray,
 
 def get_tdi_delays(orbits: OracleOrbits, y_gw: np.ndarray, t_arr: np.ndarray, tables: dict, order,
                    sampling_frequency, A_in, deps, E_in, tdi_start_ind, nthreads=1,
-                   delayed_links=None):
-    """X, Y, Z [3][N] accumulated into a zeroed (or given) buffer, like get_tdi_delays."""
-    y = np.ascontiguousarray(y_gw, dtype=np.float64).reshape(-1)
+                   delayed_links=None, wide=False):
+    """X, Y, Z [3][N] accumulated into a zeroed (or given) buffer, like get_tdi_delays;
+    wide=True: the extended-precision build (y and the outputs as long double)."""
+    rt = _real(wide)
+    y = np.ascontiguousarray(y_gw, dtype=rt).reshape(-1)
     n = y.size // 6
     t_arr = np.ascontiguousarray(t_arr, dtype=np.float64)
-    out = np.zeros(3 * n) if delayed_links is None else np.ascontiguousarray(delayed_links).reshape(-1).copy()
+    out = (np.zeros(3 * n, dtype=rt) if delayed_links is None
+           else np.ascontiguousarray(delayed_links, dtype=rt).reshape(-1).copy())
     A_in = np.ascontiguousarray(A_in, dtype=np.float64)
     E_in = np.ascontiguousarray(E_in, dtype=np.float64)
     tb = {k_: np.ascontiguousarray(v_) for k_, v_ in tables.items() if isinstance(v_, np.ndarray)}
-    st = lib().oracle_get_tdi_delays(
+    st = lib(wide).oracle_get_tdi_delays(
         _p(out), _p(y), n, n, _p(t_arr), _p(tb["unit_starts"]), _p(tb["unit_lengths"]),
         _p(tb["tdi_base_link"]), _p(tb["tdi_link_combinations"]), _p(tb["tdi_signs"]),
         _p(tb["channels"]), tables["num_units"], 3, order, sampling_frequency, 2 * order + 1,
@@ -338,16 +361,17 @@ def get_tdi_delays(orbits: OracleOrbits, y_gw: np.ndarray, t_arr: np.ndarray, ta
     return out.reshape(3, n)
 
 
-def aet(X, Y, Z):
-    X, Y, Z = (np.ascontiguousarray(a, dtype=np.float64) for a in (X, Y, Z))
+def aet(X, Y, Z, wide=False):
+    X, Y, Z = (np.ascontiguousarray(a, dtype=_real(wide)) for a in (X, Y, Z))
     A, E, T = np.zeros_like(X), np.zeros_like(X), np.zeros_like(X)
-    lib().oracle_aet(_p(X), _p(Y), _p(Z), _p(A), _p(E), _p(T), X.size)
+    lib(wide).oracle_aet(_p(X), _p(Y), _p(Z), _p(A), _p(E), _p(T), X.size)
     return A, E, T
 
 
 def response_pipeline(orbits: OracleOrbits, input_in, lam, beta, dt, order, tdi, t0=0.0,
-                      t_buffer=10000.0, tdi_chan="AET", x_base=None, nthreads=1):
-    """Projection -> TDI -> channel selection for one source, as pyResponseTDI runs it."""
+                      t_buffer=10000.0, tdi_chan="AET", x_base=None, nthreads=1, wide=False):
+    """Projection -> TDI -> channel selection for one source, as pyResponseTDI runs it
+    (wide=True: the extended-precision build; arrays returned as long double)."""
     fs = 1 / dt
     n = len(input_in)
     A_in, deps, num_A, E_in = lagrange_tables(order)
@@ -357,13 +381,14 @@ def response_pipeline(orbits: OracleOrbits, input_in, lam, beta, dt, order, tdi,
     k, u, v = sky_basis(lam, beta)
     t_data = t0 + np.arange(n) * dt
     y = get_response(orbits, input_in, t_data, k, u, v, order, fs, A_in, deps, E_in, proj_start,
-                     nthreads)
+                     nthreads, wide=wide)
     combos = stock_combinations(tdi) if isinstance(tdi, str) else tdi
     tables = tdi_tables(combos)
-    xyz = get_tdi_delays(orbits, y, t_data, tables, order, fs, A_in, deps, E_in, tdi_start, nthreads)
+    xyz = get_tdi_delays(orbits, y, t_data, tables, order, fs, A_in, deps, E_in, tdi_start, nthreads,
+                         wide=wide)
     if tdi_chan == "XYZ":
         return y, xyz
-    a, e, t = aet(*xyz)
+    a, e, t = aet(*xyz, wide=wide)
     return y, np.stack([a, e, t])
 
 

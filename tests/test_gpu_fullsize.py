@@ -1,0 +1,277 @@
+"""Every benchmark configuration at its full size, and the reference's own test case, on the GPU.
+
+* The reference test (tests/test_fastlisaresponse.py:68-151): ResponseWrapper over T = 2 yr
+  at dt = 10 s with no n_overide (N = int(2 yr / dt) = 6,311,629), its galactic binary, 1st and
+  2nd generation, A, E, T with the garbage removed. The exact FP64 form is bitwise the oracle;
+  mixed-c128 meets the reference's own CPU/GPU criterion 1 - overlap < 1e-10.
+* 1-yr source (3,153,600 samples, f0 = 5 mHz, fdot = 1e-16, 2nd generation, AET): the FP64
+  drop-in bitwise against the oracle; mixed-c128 within 1e-7 rel-L2 per channel, complex64
+  storage within 1e-6 on A and E.
+* 512-source batch (durations U[0.05, 1] yr, numpy default_rng(3)): every output finite; eight
+  sources bitwise equal to their single-source runs and to the oracle; the mixed-c128 batch
+  within 1e-6 rel-L2 of the FP64 batch on A, E and T for every source.
+* Serving: one second of Poisson arrivals at 200 req/s (1-64 sources, 2^18/2^20/2^22 samples,
+  dt 5/10 s, 30/70 1st/2nd generation, numpy default_rng(5)); a sample of requests bitwise equal
+  to the same request run alone, and the bytes in flight never above the admission budget.
+"""
+import os
+import warnings
+
+import numpy as np
+import pytest
+
+from conftest import YRSID_SI, gb_waveform, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+NTH = max(1, min(16, os.cpu_count() or 1))  # oracle threads (the host share of one GPU)
+N_YEAR = 3_153_600
+T_BUF = 1000  # tdi_start_ind at dt = 10 s, t_buffer = 1e4 s
+
+
+@pytest.fixture(scope="module")
+def torch_gpu():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+@pytest.fixture(scope="module")
+def orbits10():
+    from fastlisaresponse_amd.orbits import EqualArmlengthOrbits
+    o = EqualArmlengthOrbits()  # 10 years of tables, as the reference's default
+    o.configure(linear_interp_setup=True)
+    return o
+
+
+@pytest.fixture(scope="module")
+def oracle_orbits10(oracle_mod, orbits10):
+    return oracle_mod.OracleOrbits(*orbits10.pycppdetector_args)
+
+
+# ------------------------------------------------------------------ the reference's test case
+
+REF_PARAMS = dict(A=1.084702251e-22, f=2.35962078e-3, fdot=1.47197271e-17, iota=1.11820901,
+                  phi0=4.91128699, psi=2.3290324)
+REF_LAM, REF_BETA = 5.22979888, 0.9805742971871619
+
+
+def ref_gbwave(A, f, fdot, iota, phi0, psi, T=1.0, dt=10.0):
+    """The reference test's generator (tests/test_fastlisaresponse.py:41-63), in numpy."""
+    t = np.arange(0.0, T * YRSID_SI, dt)
+    cos2psi, sin2psi, cosiota = np.cos(2.0 * psi), np.sin(2.0 * psi), np.cos(iota)
+    fddot = 11.0 / 3.0 * fdot ** 2 / f
+    phase = 2 * np.pi * (f * t + 1.0 / 2.0 * fdot * t ** 2 + 1.0 / 6.0 * fddot * t ** 3) - phi0
+    hSp = -np.cos(phase) * A * (1.0 + cosiota * cosiota)
+    hSc = -np.sin(phase) * 2.0 * A * cosiota
+    return (hSp * cos2psi - hSc * sin2psi) + 1j * (hSp * sin2psi + hSc * cos2psi)
+
+
+def _run_reference_case(orbits, tdi_gen, **kw):
+    from fastlisaresponse_amd import ResponseWrapper
+    wrap = ResponseWrapper(ref_gbwave, 2.0, 10.0, 6, 7, t_buffer=10000.0, flip_hx=False,
+                           remove_sky_coords=True, is_ecliptic_latitude=True,
+                           remove_garbage=True, orbits=orbits, order=25, tdi=tdi_gen,
+                           tdi_chan="AET", **kw)
+    assert wrap.n == int(2.0 * YRSID_SI / 10.0) == 6_311_629
+    chans = wrap(REF_PARAMS["A"], REF_PARAMS["f"], REF_PARAMS["fdot"], REF_PARAMS["iota"],
+                 REF_PARAMS["phi0"], REF_PARAMS["psi"], REF_LAM, REF_BETA)
+    return wrap, np.stack([c.double().cpu().numpy() for c in chans])
+
+
+@pytest.mark.parametrize("tdi_gen", ["1st generation", "2nd generation"])
+def test_reference_test_case_2yr(torch_gpu, orbits10, oracle_orbits10, oracle_mod, tdi_gen):
+    from fastlisaresponse_amd import get_overlap
+    wrap, got = _run_reference_case(orbits10, tdi_gen)
+    n = wrap.n
+    T = wrap.response_model.tdi_start_ind
+    assert got.shape == (3, n - 2 * T)
+    assert np.all(np.isfinite(got))  # the reference's CPU check: no NaN
+    h = ref_gbwave(**REF_PARAMS, T=wrap.Tobs, dt=10.0)
+    assert len(h) == n
+    _, exp = oracle_mod.response_pipeline(oracle_orbits10, h, REF_LAM, REF_BETA, 10.0, 25, tdi_gen,
+                                          x_base=orbits10.x_base, nthreads=NTH)
+    np.testing.assert_array_equal(got, exp[:, T:-T])
+    # mixed-c128: the reference's CPU-vs-GPU criterion, and the per-channel error
+    _, mixed = _run_reference_case(orbits10, tdi_gen, precision="mixed-c128")
+    mm = 1.0 - get_overlap(list(exp[:, T:-T]), list(mixed), use_gpu=False)
+    assert abs(mm) < 1e-10, mm
+    for c in range(3):
+        assert rel_l2(mixed[c], exp[c, T:-T]) <= 1e-7, (c, rel_l2(mixed[c], exp[c, T:-T]))
+
+
+# ------------------------------------------------------------------ 1-yr source (plain, mixed)
+
+@pytest.fixture(scope="module")
+def year_case(orbits10, oracle_orbits10, oracle_mod):
+    h = gb_waveform(N_YEAR, 10.0, 1e-21, 5e-3, 1e-16, 0.8, 0.3, 0.5)
+    _, aet = oracle_mod.response_pipeline(oracle_orbits10, h, 1.2, 0.3, 10.0, 25, "2nd generation",
+                                          x_base=orbits10.x_base, nthreads=NTH)
+    return h, aet
+
+
+def test_year_fp64_dropin_bitwise(torch_gpu, orbits10, year_case):
+    torch = torch_gpu
+    from fastlisaresponse_amd import pyResponseTDI
+    h, aet = year_case
+    r = pyResponseTDI(0.1, N_YEAR, order=25, tdi="2nd generation", orbits=orbits10, tdi_chan="AET")
+    r.get_projections(torch.as_tensor(h, device="cuda"), 1.2, 0.3, t0=0.0, t_buffer=10000.0)
+    got = torch.stack(r.get_tdi_delays()).cpu().numpy()
+    r.check_errors()
+    np.testing.assert_array_equal(got, aet)
+
+
+@pytest.mark.parametrize("precision,tols", [("mixed-c128", (1e-7, 1e-7, 1e-7)),
+                                            ("mixed", (1e-6, 1e-6, 1e-3))])
+def test_year_mixed_vs_oracle(torch_gpu, orbits10, year_case, precision, tols):
+    """complex128 input: every channel within 1e-7 (measured ~2.5e-8, the float32 output
+    rounding); complex64 storage: A, E within 1e-6, the null channel T ~1e-4 (its storage error,
+    amplified where the signal cancels)."""
+    torch = torch_gpu
+    from fastlisaresponse_amd import pyResponseTDI
+    from fastlisaresponse_amd.batch import StorageErrorWarning
+    h, aet = year_case
+    with warnings.catch_warnings():
+        # a 5 mHz source raises no storage-error warning
+        warnings.simplefilter("error", StorageErrorWarning)
+        r = pyResponseTDI(0.1, N_YEAR, order=25, tdi="2nd generation", orbits=orbits10,
+                          tdi_chan="AET", precision=precision)
+        r.get_projections(torch.as_tensor(h, device="cuda"), 1.2, 0.3)
+    got = torch.stack(r.get_tdi_delays()).double().cpu().numpy()
+    r.check_errors()
+    assert np.all(got[:, :T_BUF] == 0.0) and np.all(got[:, -T_BUF:] == 0.0)
+    for c in range(3):
+        rc = rel_l2(got[c, T_BUF:-T_BUF], aet[c, T_BUF:-T_BUF])
+        assert rc <= tols[c], (precision, c, rc)
+
+
+# ------------------------------------------------------------------ 512-source batch
+
+def batch512_params(num_sources=512, seed=3, dt=10.0):
+    """The 512-source batch: durations U[0.05, 1] yr, sky uniform on the sphere, psi and phi0
+    U[0, 2 pi), cos i U[-1, 1], f0 log-U[1e-4, 1e-2] Hz, fdot log-U[1e-19, 1e-15] Hz/s, log10 A
+    U[-23, -21]; drawn in this order from numpy default_rng(seed)."""
+    rng = np.random.default_rng(seed)
+    lengths = (rng.uniform(0.05, 1.0, num_sources) * YRSID_SI / dt).astype(np.int64)
+    lam = rng.uniform(0, 2 * np.pi, num_sources)
+    beta = np.arcsin(rng.uniform(-1, 1, num_sources))
+    psi = rng.uniform(0, 2 * np.pi, num_sources)
+    phi0 = rng.uniform(0, 2 * np.pi, num_sources)
+    ci = rng.uniform(-1, 1, num_sources)
+    f0 = 10 ** rng.uniform(-4, -2, num_sources)
+    fd = 10 ** rng.uniform(-19, -15, num_sources)
+    amp = 10 ** rng.uniform(-23, -21, num_sources)
+    gb = np.stack([amp, f0, fd, np.arccos(ci), phi0, psi], axis=1)
+    return lengths, lam, beta, gb
+
+
+@pytest.fixture(scope="module")
+def batch512(torch_gpu, orbits10):
+    torch = torch_gpu
+    from fastlisaresponse_amd.batch import BatchedResponse
+    from fastlisaresponse_amd.waveforms import generate_gb
+    lengths, lam, beta, gb = batch512_params()
+    h, _ = generate_gb(gb, lengths, 10.0, fmt="complex128")
+    br = BatchedResponse(0.1, tdi="2nd generation", orbits=orbits10, tdi_chan="AET")
+    plan = br.plan(h, lengths, lam, beta)
+    plan.run()
+    plan.check_errors()
+    out64 = plan.out_all
+    return dict(h=h, lengths=lengths, lam=lam, beta=beta, plan=plan, out64=out64)
+
+
+SAMPLED = [0, 77, 150, 201, 333, 402, 480, 511]
+
+
+def test_batch512_fp64_bitwise(torch_gpu, orbits10, oracle_orbits10, oracle_mod, batch512):
+    torch = torch_gpu
+    from fastlisaresponse_amd import pyResponseTDI
+    b = batch512
+    assert int(b["lengths"].sum()) == b["plan"].samples > 8e8
+    assert bool(torch.isfinite(b["out64"]).all())
+    for s in SAMPLED:
+        n = int(b["lengths"][s])
+        o = int(b["plan"].offsets[s])
+        hs = b["h"][o:o + n]
+        got = torch.stack(b["plan"].channels(s))
+        r = pyResponseTDI(0.1, n, order=25, tdi="2nd generation", orbits=orbits10, tdi_chan="AET")
+        r.get_projections(hs, b["lam"][s], b["beta"][s])
+        assert torch.equal(got, torch.stack(r.get_tdi_delays())), s
+        _, exp = oracle_mod.response_pipeline(oracle_orbits10, hs.cpu().numpy(), b["lam"][s],
+                                              b["beta"][s], 10.0, 25, "2nd generation",
+                                              x_base=orbits10.x_base, nthreads=NTH)
+        np.testing.assert_array_equal(got.cpu().numpy(), exp, err_msg=f"source {s}")
+
+
+def test_batch512_mixed_c128_within_1e6(torch_gpu, orbits10, batch512):
+    torch = torch_gpu
+    from fastlisaresponse_amd.batch import BatchedResponse
+    b = batch512
+    br = BatchedResponse(0.1, tdi="2nd generation", orbits=orbits10, tdi_chan="AET",
+                         precision="mixed-c128")
+    pm = br.plan(b["h"], b["lengths"], b["lam"], b["beta"])
+    pm.tune(use_cache=False, reps=1)
+    pm.run()
+    pm.check_errors()
+    worst = np.zeros(3)
+    for s in range(len(b["lengths"])):
+        n = int(b["lengths"][s])
+        ref = torch.stack(b["plan"].channels(s))[:, T_BUF:n - T_BUF]
+        got = torch.stack(pm.channels(s))[:, T_BUF:n - T_BUF].double()
+        rel = ((got - ref).norm(dim=1) / ref.norm(dim=1)).cpu().numpy()
+        worst = np.maximum(worst, rel)
+        assert np.all(rel <= 1e-6), (s, rel)
+    assert np.all(worst > 0.0)  # float32 outputs: not trivially equal
+
+
+# ------------------------------------------------------------------ serving at 200 req/s
+
+def test_serving_200_per_s_fullsize(torch_gpu):
+    torch = torch_gpu
+    from fastlisaresponse_amd.batch import BatchedResponse
+    from fastlisaresponse_amd.orbits import EqualArmlengthOrbits
+    from fastlisaresponse_amd.serving import ResponseServer, poisson_requests, run_open_loop
+    from fastlisaresponse_amd.waveforms import generate_gb
+
+    orb = EqualArmlengthOrbits(t_max=1.5 * YRSID_SI)  # 2^22 samples at dt = 10 s: 1.33 yr
+    orb.configure(linear_interp_setup=True)
+    keys = [(dt, g) for dt in (5.0, 10.0) for g in ("1st generation", "2nd generation")]
+
+    class Watched(ResponseServer):
+        peak = 0
+
+        def admit(self):
+            started = super().admit()
+            assert self.in_flight_bytes <= self.hbm_budget
+            Watched.peak = max(Watched.peak, self.in_flight_bytes)
+            return started
+
+    server = Watched(orb, keys, num_streams=2)
+    times, reqs = poisson_requests(np.random.default_rng(5), 200.0, 1.0)
+    assert 150 <= len(reqs) <= 250
+    lengths = {s.num_pts for r in reqs for s in r.sources}
+    assert lengths == {1 << 18, 1 << 20, 1 << 22}
+    gens = [r.tdi for r in reqs]
+    assert 0.15 < gens.count("1st generation") / len(gens) < 0.45
+    sample = reqs[:: max(1, len(reqs) // 6)][:6]
+    for r in sample:
+        r.keep = True
+    stats = run_open_loop(server, times, reqs, drain_timeout=120.0)
+    assert stats["requests"] == len(reqs)
+    assert 0 < Watched.peak <= server.hbm_budget
+    offered = sum(r.samples for r in reqs)
+    assert stats["samples"] == offered
+    for req in sample:
+        params = np.array([s.gb for s in req.sources])
+        lens = [s.num_pts for s in req.sources]
+        hin, scales = generate_gb(params, lens, req.dt, fmt="complex128")
+        br = BatchedResponse(1.0 / req.dt, tdi=req.tdi, orbits=orb, precision="mixed-c128")
+        plan = br.plan(hin, lens, [s.lam for s in req.sources], [s.beta for s in req.sources],
+                       scales=list(scales))
+        plan.run()
+        plan.check_errors()
+        assert torch.equal(req.result, plan.out_all), req.rid
+        assert bool(torch.isfinite(req.result).all())
+        del plan, hin
+    server.close()

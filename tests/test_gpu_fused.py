@@ -509,3 +509,109 @@ def test_fused_window_error_is_reported_not_faulted(torch_gpu, orbits, precision
     assert np.isnan(o[:, 3:40]).all()           # the first tile's samples: window error
     assert np.isfinite(o[:, 5000:15000]).all()  # samples far from the ends are computed
     br.model.cpp_response.check_errors()        # the error word was cleared
+
+
+@pytest.mark.parametrize("kind", ["mixed-c128", "fused-fp64-exact"])
+def test_bound_sources_retile_when_chains_lengthen(torch_gpu, orbits, kind):
+    """A batch bound under 1st-generation tables whose response switches to 2nd generation: the
+    longer delay chains widen the TDI halo and shorten the tile, so the bound sources are
+    re-tiled at the next run -- every sample is computed, bitwise as a plan made for the new
+    table (a source of 3e5 samples at dt = 10 s spans ~650 tiles, so a stale tiling would leave
+    thousands of samples unwritten)."""
+    torch = torch_gpu
+    from fastlisaresponse_amd.batch import BatchedResponse
+    from fastlisaresponse_amd.tables import stock_tdi, tdi_tables
+    rng = np.random.default_rng(61)
+    hs, params = _sources(rng, [300000, 50000])
+    kw = dict(precision="mixed-c128") if kind == "mixed-c128" else dict(fused=True)
+    br = BatchedResponse(0.1, tdi="1st generation", orbits=orbits, **kw)
+    plan = _plan(torch, br, hs, params)
+    plan.run()
+    plan.check_errors()
+    tabs = tdi_tables(stock_tdi("2nd generation"))
+    br.model.cpp_response.set_tdi(tabs["unit_starts"], tabs["unit_lengths"], tabs["tdi_base_links"],
+                                  tabs["tdi_link_combinations"], tabs["tdi_signs"],
+                                  tabs["channels"], tabs["num_units"], 3)
+    plan.out_all.fill_(float("nan"))
+    plan.run()
+    plan.check_errors()
+    b2 = BatchedResponse(0.1, tdi="2nd generation", orbits=orbits, **kw)
+    p2 = _plan(torch, b2, hs, params)
+    p2.run()
+    p2.check_errors()
+    assert not bool(torch.isnan(plan.out_all).any())
+    assert torch.equal(plan.out_all, p2.out_all)
+
+
+def test_mixed_dropin_rebinds_across_streams(torch_gpu, orbits):
+    """pyResponseTDI's mixed form rebinds its reserved batch per call on the caller's current
+    stream: a call on a second stream, made while the first stream's launch may still run, waits
+    for that launch before replacing the source table. Both results equal fresh instances'."""
+    torch = torch_gpu
+    from fastlisaresponse_amd import pyResponseTDI
+    n = 1 << 20
+    hs = [torch.as_tensor(gb_waveform(n, 10.0, 1e-21, f, 1e-17, 0.8, 0.3, 0.5), device="cuda")
+          for f in (3e-3, 7e-3)]
+    r = pyResponseTDI(0.1, n, orbits=orbits, tdi="2nd generation", tdi_chan="AET",
+                      precision="mixed-c128")
+    outs = []
+    torch.cuda.synchronize()
+    for h, st in zip(hs, (torch.cuda.Stream(), torch.cuda.Stream())):
+        with torch.cuda.stream(st):
+            r.get_projections(h, 1.2, 0.3)
+            outs.append(torch.stack(r.get_tdi_delays()))
+    torch.cuda.synchronize()
+    for h, got in zip(hs, outs):
+        f = pyResponseTDI(0.1, n, orbits=orbits, tdi="2nd generation", tdi_chan="AET",
+                          precision="mixed-c128")
+        f.get_projections(h, 1.2, 0.3)
+        assert torch.equal(got, torch.stack(f.get_tdi_delays()))
+    r.check_errors()
+
+
+@pytest.mark.parametrize("precision", ["mixed-c128", "mixed"])
+def test_fused_mixed_graph_replays_across_rebind(torch_gpu, orbits, precision):
+    """The headline form in a HIP graph: run() of a bound mixed-precision fused batch (composite
+    filters built at bind) captured once; replays equal the eager run, with eager work and
+    synchronisation between them; after a rebind to new parameters and new inputs of the same
+    lengths (new start times, so the filters are rebuilt outside the graph) the same graph
+    replays the new batch, equal to a fresh plan's eager run."""
+    torch = torch_gpu
+    from fastlisaresponse_amd.batch import BatchedResponse, pack_mixed
+    rng = np.random.default_rng(67)
+    lengths = [40000, 9000, 25000]
+    hs, params = _sources(rng, lengths)
+    hs2, params2 = _sources(rng, lengths)
+    br = BatchedResponse(0.1, tdi="2nd generation", orbits=orbits, tdi_chan="AET",
+                         precision=precision, storage_check="off")
+
+    def stored(hlist):
+        h = torch.as_tensor(np.concatenate(hlist), device="cuda")
+        return pack_mixed(h, lengths) if precision == "mixed" else (h, None)
+
+    h1, sc1 = stored(hs)
+    plan = _plan(torch, br, hs, params, scales=sc1, h_all=h1)
+    plan.run()
+    torch.cuda.synchronize()
+    eager = plan.out_all.clone()
+    g = plan.graph()
+    for rep in range(3):
+        plan.out_all.fill_(float("nan"))  # eager work between the replays
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(plan.out_all, eager), rep
+    # rebind: new sky positions, start times and input contents in a new tensor of the same size
+    h2, sc2 = stored(hs2)
+    plan.rebind([p[0] for p in params2], [p[1] for p in params2], [p[2] for p in params2],
+                scales=sc2, h_all=h2)
+    fresh = _plan(torch, br, hs2, params2, scales=sc2, h_all=h2)
+    fresh.run()
+    for rep in range(2):
+        plan.out_all.fill_(float("nan"))
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(plan.out_all, fresh.out_all), rep
+    assert not torch.equal(eager, fresh.out_all)
+    plan.check_errors()

@@ -285,6 +285,59 @@ def test_dropin_wraps_match_oracle(gpu, orbits, oracle_orbits, oracle_mod):
     np.testing.assert_array_equal(y_np.reshape(6, n), y_exp)
 
 
+def test_reference_host_flow_with_numpy_xp(gpu, orbits, oracle_orbits, oracle_mod):
+    """The reference's host code with xp = numpy (the backend INTEGRATION.md sketches): its
+    get_projections (response.py:526-557) and get_tdi_delays (:592-657) call sequence, arrays
+    made with numpy as it makes them, through the classes of backend.backend_methods(); the
+    projections, X, Y, Z and A, E, T are bitwise the oracle's."""
+    from fastlisaresponse_amd.backend import backend_methods
+    from fastlisaresponse_amd.tables import buffer_indices, lagrange_tables, stock_tdi, tdi_tables
+    from fastlisaresponse_amd.utility import AET
+    xp = np
+    m = backend_methods()
+    cpp_orbits = m["OrbitsWrap"](*orbits.pycppdetector_args)
+    cpp_response = m["LISAResponseWrap"](cpp_orbits)
+    order, dt, t0, n = 25, 10.0, 2.0e6, 6000
+    fs = 1 / dt
+    A_in, deps, E_in = lagrange_tables(order)
+    A_in, E_in = xp.asarray(A_in), xp.asarray(E_in)
+    tabs = tdi_tables(stock_tdi("2nd generation"))
+    tdi_base_links = xp.asarray(tabs["tdi_base_links"]).astype(xp.int32)
+    tdi_link_combinations = xp.asarray(tabs["tdi_link_combinations"]).astype(xp.int32)
+    tdi_signs = xp.asarray(tabs["tdi_signs"]).astype(xp.float64)
+    channels = xp.asarray(tabs["channels"]).astype(xp.int32)
+    tdi_operation_index = xp.asarray(tabs["tdi_operation_index"]).astype(xp.int32)
+    T, _, _, P = buffer_indices(fs, order, 10000.0, orbits.x_base)
+    from fastlisaresponse_amd.tables import sky_basis
+    k, u, v = sky_basis(0.7, -0.9)
+    k_in, u_in, v_in = xp.asarray(k), xp.asarray(u), xp.asarray(v)
+    input_in = xp.asarray(gb_waveform(n, dt, t0=t0))
+    t_data = t0 + xp.arange(len(input_in)) * dt
+    y_gw = xp.zeros((6 * n,), dtype=xp.float64)
+    cpp_response.get_response_wrap(y_gw, t_data, k_in, u_in, v_in, dt, len(input_in), input_in,
+                                   len(input_in), order, fs, 2 * order + 1, A_in, deps,
+                                   len(A_in), E_in, P)
+    delayed_links_flat = xp.zeros((3, n), dtype=xp.float64).flatten()
+    t_data = t0 + xp.arange(n) * dt
+    num_units = int(tdi_operation_index.max() + 1)
+    _, unit_starts, unit_lengths = np.unique(tdi_operation_index, return_index=True,
+                                             return_counts=True)
+    cpp_response.get_tdi_delays_wrap(delayed_links_flat, y_gw.reshape(6, -1).flatten().copy(), n,
+                                     n, t_data, unit_starts.astype(np.int32),
+                                     unit_lengths.astype(np.int32), tdi_base_links,
+                                     tdi_link_combinations, tdi_signs, channels, num_units, 3,
+                                     order, fs, 2 * order + 1, A_in, deps, len(A_in), E_in, T)
+    X, Y, Z = delayed_links_flat.reshape(3, -1)
+    A, E, Tc = AET(X, Y, Z)
+    y_exp = oracle_mod.get_response(oracle_orbits, input_in, t_data, k, u, v, order, fs,
+                                    np.asarray(A_in), deps, np.asarray(E_in), P, 8)
+    xyz_exp = oracle_mod.get_tdi_delays(oracle_orbits, y_exp, t_data, _oracle_tables(tabs), order,
+                                        fs, np.asarray(A_in), deps, np.asarray(E_in), T, 8)
+    np.testing.assert_array_equal(y_gw.reshape(6, n), y_exp)
+    np.testing.assert_array_equal(np.stack([X, Y, Z]), xyz_exp)
+    np.testing.assert_array_equal(np.stack([A, E, Tc]), np.stack(oracle_mod.aet(*xyz_exp)))
+
+
 def test_dropin_length_checks(gpu, orbits):
     import torch
     _, rw, (A, deps, E), _ = _setup(gpu, orbits)

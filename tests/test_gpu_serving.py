@@ -155,3 +155,39 @@ def test_parameter_requests_generated_on_device(torch_gpu, orbits, precision):
         plan.check_errors()
         assert torch.equal(req.result, plan.out_all), req.rid
     server.close()
+
+
+def test_unsynchronised_producer(torch_gpu, orbits):
+    """Clients hand over device inputs still being produced on their own stream, without any
+    synchronisation: the server's copy waits for the point of submission on that stream, so
+    every result equals the request run alone on the finished input."""
+    torch = torch_gpu
+    from fastlisaresponse_amd.serving import Request, ResponseServer, SourceSpec, gb_pool
+    pool = gb_pool(3, 1 << 16, 10.0, seed=3, dtype=torch.complex128)
+    server = ResponseServer(orbits, KEYS, num_streams=2, keep_results=True)
+    producer = torch.cuda.Stream()
+    reqs = []
+    rng = np.random.default_rng(19)
+    with torch.cuda.stream(producer):
+        for rid in range(6):
+            key = KEYS[rid % len(KEYS)]
+            h = torch.full((1 << 16,), float("nan"), dtype=torch.complex128, device="cuda")
+            x = torch.ones(2048, 2048, dtype=torch.float64, device="cuda")
+            for _ in range(6):  # several milliseconds of work ahead of the input's last write
+                x = (x @ x) / 2048.0
+            h.copy_(pool[rid % 3] + 0.0 * x[0, 0])
+            req = Request(rid, key[0], key[1], [SourceSpec(h=h, lam=float(rng.uniform(0, 6)),
+                                                           beta=float(rng.uniform(-1, 1)),
+                                                           scale=1e-21)])
+            server.submit(req)  # no synchronisation: the producer is still running
+            reqs.append(req)
+            del h, x
+            server.step()
+    while not server.idle():
+        server.step()
+    server.check_errors()
+    torch.cuda.synchronize()
+    for req in reqs:
+        assert not bool(torch.isnan(req.result).any()), req.rid
+        assert torch.equal(req.result, _alone(torch, orbits, req, "mixed-c128")), req.rid
+    server.close()
