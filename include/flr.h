@@ -223,6 +223,21 @@ int flr_batch_set_launch(flr_batch *batch, int tile_samples, int grid_per_cu);
 int64_t flr_batch_samples(const flr_batch *batch);
 void flr_batch_destroy(flr_batch *batch);
 
+/* Allocation-free fused run. No counterpart in the reference: for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Computes what flr_fused_create(response, sources, num_sources, h_all, out_all, dt,
+ * sampling_frequency, flags) followed by flr_batch_run would (same kernels, same bits), with the
+ * launch shape of flr_batch_set_launch(tile_samples, grid_per_cu) (0, 0: the defaults), but
+ * allocates nothing: the source table, its tile prefix and the composite TDI filters go to one
+ * fixed device array per GPU (under 512 KiB, held until the process ends), and the source
+ * records reach it as kernel arguments. At most 512 sources per call (FLR_ERR_UNSUPPORTED
+ * beyond). Everything is enqueued on `stream`; device errors are latched as for the other
+ * stream-ordered calls (flr_check_errors). Calls on one GPU share that array, so calls that
+ * overlap on one GPU (concurrent streams or host threads) race: order them. */
+int flr_fused_run_static(flr_response *response, const flr_source *sources, int num_sources,
+                         const void *h_all, void *out_all, double dt, double sampling_frequency,
+                         int flags, int tile_samples, int grid_per_cu, void *stream);
+
 /* ------------------------------------------------------------------ waveforms
  * No counterpart in the reference library (its tests generate galactic binaries with numpy,
  * tests/test_fastlisaresponse.py:41-63): the same generator on the device, writing straight into

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1205,3 +1206,159 @@ int flr_response_set_exact(flr_response *r, int exact)
 
 }  // extern "C"
 
+// ====================================================================== allocation-free run
+// flr_fused_run_static: the fused batch of flr_fused_create + flr_batch_run without any device
+// allocation. Its device tables (source records, tile prefix, composite TDI filters) live in
+// one file-scope device array per GPU, of fixed size: kStaticSources sources at most.
+
+namespace {
+
+constexpr int kStaticSources = 512;
+constexpr size_t kStaticTileOff = (sizeof(FusedSource) * kStaticSources + 255) & ~size_t(255);
+constexpr size_t kStaticLtiOff =
+    (kStaticTileOff + sizeof(int) * (kStaticSources + 1) + 255) & ~size_t(255);
+constexpr size_t kStaticBytes = kStaticLtiOff + sizeof(double) * kLtiRecord * kStaticSources;
+static_assert(kStaticBytes <= 512 * 1024, "the static scratch is meant to stay under 512 KiB");
+
+}  // namespace
+
+// The scratch array. In host code its name is a host-side stand-in: kernels only ever receive
+// the device address returned by hipGetSymbolAddress (static_scratch_address).
+static __device__ __align__(256) unsigned char scratch[kStaticBytes];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+void *g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Device address of `scratch` on `device`, which must be current: looked up on the first call
+// on that device and kept (each GPU holds its own copy of the array at its own address).
+int static_scratch_address(int device, unsigned char **ptr)
+{
+    if (device < 0 || device >= kMaxScratchDevices)
+        return fail(FLR_ERR_UNSUPPORTED, "device index beyond the static scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        if (!p) return fail(FLR_ERR_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+        g_scratch_ptr[device] = p;
+    }
+    *ptr = static_cast<unsigned char *>(g_scratch_ptr[device]);
+    return FLR_OK;
+}
+
+// Source records travel to the device as kernel arguments, kStageChunk per launch (no host
+// staging buffer and no copy from host memory, so the sequence can be captured into a graph).
+constexpr int kStageChunk = 16;
+
+struct StageChunk {
+    FusedSource src[kStageChunk];
+    int tile_end[kStageChunk];  // tile prefix after source first + k
+    int first;                  // table index of src[0]
+    int count;                  // records used, <= kStageChunk
+};
+static_assert(sizeof(StageChunk) <= 2048, "kernel argument block stays small");
+
+__global__ void stage_sources_kernel(StageChunk c, FusedSource *table, int *prefix)
+{
+    static_for<kStageChunk>([&](auto k) {
+        if ((int)threadIdx.x == k && k < c.count) {
+            table[c.first + k] = c.src[k];
+            prefix[c.first + k + 1] = c.tile_end[k];
+        }
+    });
+    if (threadIdx.x == 0 && c.first == 0) prefix[0] = 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int flr_fused_run_static(flr_response *r, const flr_source *sources, int num_sources,
+                         const void *h_all, void *out_all, double dt, double sampling_frequency,
+                         int flags, int tile_samples, int grid_per_cu, void *stream)
+{
+    if (!r || !r->orbits) return fail(FLR_ERR_NO_ORBITS, "Must add orbits with add_orbit_information method.");
+    if (!r->lag_set || !r->tdi_set) return fail(FLR_ERR_INVALID_ARGUMENT, "Lagrange and TDI tables must be set");
+    if (r->order != 25) return fail(FLR_ERR_UNSUPPORTED, "fused batches are built for order 25");
+    if (!sources || num_sources < 1) return fail(FLR_ERR_INVALID_ARGUMENT, "no sources");
+    if (num_sources > kStaticSources)
+        return fail(FLR_ERR_UNSUPPORTED, "flr_fused_run_static takes at most " +
+                                             std::to_string(kStaticSources) +
+                                             " sources per call (its device tables have a fixed size)");
+    if (!h_all || !out_all) return fail(FLR_ERR_INVALID_ARGUMENT, "NULL packed array");
+    if (!(sampling_frequency > 0.0) || !(dt > 0.0)) return fail(FLR_ERR_INVALID_ARGUMENT, "dt and sampling_frequency must be > 0");
+    if ((flags & FLR_FLAG_MIXED) && (flags & FLR_FLAG_EXACT))
+        return fail(FLR_ERR_UNSUPPORTED, "FLR_FLAG_EXACT needs FP64 storage (no FLR_FLAG_MIXED)");
+    if ((flags & (FLR_FLAG_Q31 | FLR_FLAG_C128)) && !(flags & FLR_FLAG_MIXED))
+        return fail(FLR_ERR_INVALID_ARGUMENT, "FLR_FLAG_Q31 and FLR_FLAG_C128 select a mixed-precision input format");
+    if ((flags & FLR_FLAG_Q31) && (flags & FLR_FLAG_C128))
+        return fail(FLR_ERR_INVALID_ARGUMENT, "FLR_FLAG_Q31 and FLR_FLAG_C128 exclude each other");
+    if (grid_per_cu < 0 || grid_per_cu > 64)
+        return fail(FLR_ERR_INVALID_ARGUMENT, "grid_per_cu must be in [1, 64] (0: the default 32)");
+    const bool mixed = (flags & FLR_FLAG_MIXED) != 0;
+    int passes = kFusedPasses;
+    if (tile_samples != 0) {
+        passes = tile_samples / kBlock;
+        const bool ok = tile_samples % kBlock == 0 &&
+                        (mixed ? (passes == 1 || passes == 2 || passes == 4) : passes == kFusedPasses);
+        if (!ok)
+            return fail(FLR_ERR_UNSUPPORTED, mixed ? "mixed fused tiles are 256, 512 or 1024 samples"
+                                                   : "FP64 fused tiles are 512 samples");
+    }
+    DeviceGuard g(r->device);
+    hipStream_t s = (hipStream_t)stream;
+    ProjConfig pc;
+    TdiConfig tc;
+    StockTdi stock;
+    int st = make_fused_config(r, dt, sampling_frequency, flags, passes, pc, tc, stock);
+    if (st) return st;
+    // validate every source before anything is enqueued
+    const int zero_edges = (flags & FLR_FLAG_ZERO_EDGES) ? 1 : 0;
+    int64_t tiles = 0;
+    for (int q = 0; q < num_sources; ++q) {
+        FusedSource f;
+        if ((st = fused_source_record(sources[q], q, flags, h_all, out_all, f))) return st;
+        tiles += tiles_for(f.num_pts, f.tdi_start, zero_edges, tc.tile);
+        if (tiles > INT32_MAX) return fail(FLR_ERR_UNSUPPORTED, "batch has too many tiles");
+    }
+    unsigned char *base = nullptr;
+    if ((st = static_scratch_address(r->device, &base))) return st;
+    FusedSource *d_src = reinterpret_cast<FusedSource *>(base);
+    int *d_prefix = reinterpret_cast<int *>(base + kStaticTileOff);
+    double *d_lti = reinterpret_cast<double *>(base + kStaticLtiOff);
+    StageChunk chunk;
+    int prefix = 0;
+    for (int first = 0; first < num_sources; first += kStageChunk) {
+        std::memset(&chunk, 0, sizeof(chunk));
+        chunk.first = first;
+        chunk.count = std::min(kStageChunk, num_sources - first);
+        for (int k = 0; k < chunk.count; ++k) {
+            (void)fused_source_record(sources[first + k], first + k, flags, h_all, out_all,
+                                      chunk.src[k]);
+            prefix += tiles_for(chunk.src[k].num_pts, chunk.src[k].tdi_start, zero_edges, tc.tile);
+            chunk.tile_end[k] = prefix;
+        }
+        hipLaunchKernelGGL(stage_sources_kernel, dim3(1), dim3(64), 0, s, chunk, d_src, d_prefix);
+        FLR_HIP(hipGetLastError());
+    }
+    Batch<FusedSource> bf{d_src, d_prefix, num_sources, prefix};
+    const LagrangeView lg = r->lagrange();
+    const OrbitView orb = r->orbits->view();
+    if (mixed) {
+        tc.lti_fir = d_lti;
+        hipError_t e = launch_lti_filters(tc, bf, orb, lg, stock, s);
+        if (e != hipSuccess) return hip_fail(e, "composite filter launch");
+    }
+    hipError_t e = launch_fused_batch(pc, tc, bf, orb, lg, (flags & FLR_FLAG_EXACT) != 0, mixed,
+                                      (flags & FLR_FLAG_Q31) ? kInQ31
+                                      : (flags & FLR_FLAG_C128) ? kInC128 : kInC64,
+                                      passes, grid_per_cu != 0 ? grid_per_cu : 32, stock, s);
+    if (e != hipSuccess) return hip_fail(e, "fused batch launch");
+    return FLR_OK;
+}
+
+}  // extern "C"

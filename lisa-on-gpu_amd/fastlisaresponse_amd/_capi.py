@@ -58,6 +58,7 @@ SIGNATURES = {
     "flr_batch_set_launch": (_ci, [_vp, _ci, _ci]),
     "flr_batch_samples": (ctypes.c_int64, [_vp]),
     "flr_batch_destroy": (None, [_vp]),
+    "flr_fused_run_static": (_ci, [_vp, _vp, _ci, _vp, _vp, _cd, _cd, _ci, _ci, _ci, _vp]),
     "flr_gb_generate": (_ci, [_vp, _ci, ctypes.c_int64, _cd, _ci, _vp, _vp]),
     "flr_response_set_exact": (_ci, [_vp, _ci]),
 }
