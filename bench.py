@@ -27,8 +27,11 @@ Also on the same JSON line:
                 mixed-precision samples/s (complex64, fixed-point and complex128 inputs), and
                 each form's per-source rel-L2 against the FP64 exact batch
   latency       p50/p99 of a short open-loop serving run: Poisson arrivals at 200 req/s of
-                1..64-source requests (2^18/2^20/2^22 samples, dt 5/10 s, 30/70 1st/2nd gen)
-  parity        rel-L2 / max-abs of A, E, T against the FP64 oracle on the full workload
+                1..64-source requests (2^18/2^20/2^22 samples, dt 5/10 s, 30/70 1st/2nd gen);
+                latency_curve: the same traffic at 100, 150 and 200 req/s
+  parity        rel-L2 / max-abs of A, E, T against the FP64 oracle on the full workload, and
+                every form's rel-L2 against the oracle built with extended-precision values
+                (rel_l2_AET_vs_wide; fp64_oracle: the FP64 oracle's own, its rounding floor)
   cpu_baseline  the FP64 CPU oracle (1 thread, the reference CPU path's serial arithmetic) on
                 a bounded prefix of the same workload; `all_cores`: the same oracle with OpenMP
                 over samples (up to 16 threads, the host share of one GPU) on the whole source
@@ -166,18 +169,10 @@ def batch_run(num_sources, steps, seed=3):
     fd = 10 ** rng.uniform(-19, -15, num_sources)
     amp = 10 ** rng.uniform(-23, -21, num_sources)
     total = int(lengths.sum())
-    h = torch.empty(total, dtype=torch.complex128, device="cuda")
-    o = 0
-    for s in range(num_sources):  # GB generator of the reference's test, on the device
-        n = int(lengths[s])
-        t = torch.arange(n, dtype=torch.float64, device="cuda") * DT
-        fdd = 11.0 / 3.0 * fd[s] ** 2 / f0[s]
-        ph = 2 * np.pi * (f0[s] * t + 0.5 * fd[s] * t * t + fdd * t ** 3 / 6.0) - phi0[s]
-        hp0 = -torch.cos(ph) * amp[s] * (1.0 + ci[s] ** 2)
-        hc0 = -torch.sin(ph) * 2.0 * amp[s] * ci[s]
-        c2, s2 = np.cos(2 * psi[s]), np.sin(2 * psi[s])
-        h[o:o + n] = torch.complex(hp0 * c2 - hc0 * s2, hp0 * s2 + hc0 * c2)
-        o += n
+    # the reference test's galactic-binary generator, on the device (waveforms.generate_gb)
+    from fastlisaresponse_amd.waveforms import generate_gb
+    gb = np.stack([amp, f0, fd, np.arccos(ci), phi0, psi], axis=1)
+    h, _ = generate_gb(gb, lengths, DT, fmt="complex128")
     torch.cuda.synchronize()
     res = {"sources": num_sources, "samples": total, "seed": seed,
            "workload": "512 galactic binaries, durations U[0.05,1] yr at dt=10 s, 2nd-gen AET"}
@@ -217,7 +212,8 @@ def batch_run(num_sources, steps, seed=3):
     from fastlisaresponse_amd.batch import pack_q31
     h32, scales = pack_mixed(h, lengths)
     hq, qscales = pack_q31(h, lengths)  # both forms quantise the same FP64 waveforms
-    mb = BatchedResponse(1 / DT, tdi="2nd generation", orbits=orb, precision="mixed")
+    mb = BatchedResponse(1 / DT, tdi="2nd generation", orbits=orb, precision="mixed",
+                         storage_check="off")
     pm = mb.plan(h32, lengths, lam, beta, scales=scales)
     tune = pm.tune(use_cache=False, reps=2)
     res["mixed_tune"] = {"best": tune["best"], "ms": tune["ms"]}
@@ -234,7 +230,8 @@ def batch_run(num_sources, steps, seed=3):
     res["mixed_rel_l2_A_median_by_f0"] = by_f
     del pm, h32
     # the same batch with 32-bit fixed-point input
-    qb = BatchedResponse(1 / DT, tdi="2nd generation", orbits=orb, precision="mixed-q31")
+    qb = BatchedResponse(1 / DT, tdi="2nd generation", orbits=orb, precision="mixed-q31",
+                         storage_check="off")
     pq = qb.plan(hq, lengths, lam, beta, scales=qscales)
     pq.tune(use_cache=False, reps=1)
     wall, ms = time_plan(pq)
@@ -255,13 +252,16 @@ def batch_run(num_sources, steps, seed=3):
 
 
 def serving_run(seconds, rate, seed=5):
+    """Poisson arrivals at `rate` req/s for `seconds` s (numpy default_rng(seed)), served with
+    the default admission budget (90% of the device's memory), drained; latency and
+    throughput."""
     import numpy as np
     from fastlisaresponse_amd.orbits import EqualArmlengthOrbits
     from fastlisaresponse_amd.serving import ResponseServer, poisson_requests, run_open_loop
     orb = EqualArmlengthOrbits(t_max=1.5 * YRSID_SI)
     orb.configure(linear_interp_setup=True)
     keys = [(dt, g) for dt in (5.0, 10.0) for g in ("1st generation", "2nd generation")]
-    server = ResponseServer(orb, keys, hbm_budget=64 << 30, num_streams=2)
+    server = ResponseServer(orb, keys, num_streams=2)
     # warm-up: a few small requests, outside the measurement
     wt, wr = poisson_requests(np.random.default_rng(seed + 1), 50.0, 0.2,
                               lengths=(1 << 18,), max_sources=4)
@@ -275,10 +275,11 @@ def serving_run(seconds, rate, seed=5):
     server.close()
     stats.pop("queue_trace", None)
     stats.update({"arrival_rate_per_s": rate, "arrival_seconds": seconds,
+                  "hbm_budget_bytes": server.hbm_budget,
                   "workload": "Poisson arrivals, 1-64 sources/request, N in {2^18, 2^20, 2^22}, "
                               "dt in {5, 10} s, 30/70 1st/2nd gen, galactic-binary parameters "
                               "generated on the device as complex128, fused kernel with "
-                              "float32 outputs, 2 streams, 64 GiB admission budget"})
+                              "float32 outputs, 2 streams, admission budget 90% of HBM"})
     return stats
 
 
@@ -495,8 +496,20 @@ def main() -> int:
         nth = max(1, min(16, os.cpu_count() or 1))
         _, aet_ref = orc.response_pipeline(oo, h_host, lam, beta, DT, ORDER, args.tdi,
                                            x_base=orbits.x_base, nthreads=nth)
-        par = {"vs": "FP64 CPU oracle, full workload"}
+        # the same arithmetic with extended-precision values (time/delay/index arithmetic kept):
+        # measures how far every FP64 form, the oracle included, is from its exact value
+        _, aet_wide = orc.response_pipeline(oo, h_host, lam, beta, DT, ORDER, args.tdi,
+                                            x_base=orbits.x_base, nthreads=nth, wide=True)
+        aet_wide = aet_wide.astype(np.float64)
+        par = {"vs": "FP64 CPU oracle, full workload; rel_l2_AET_vs_wide: against the same "
+                     "arithmetic in x87 extended precision (the FP64 rounding floor)"}
         T = int(T_BUFFER / DT)
+
+        def vs_wide(got):
+            return [float(np.linalg.norm(got[c, T:-T] - aet_wide[c, T:-T]) /
+                          np.linalg.norm(aet_wide[c, T:-T])) for c in range(3)]
+
+        par["fp64_oracle"] = {"rel_l2_AET_vs_wide": vs_wide(aet_ref)}
         for name, out in (*((p.replace("-", "_"), o) for p, o in outs.items()),
                           ("fp64", out_fp64),
                           ("fp64_fast", out_fast), ("fp64_fused", out_fused64)):
@@ -507,7 +520,8 @@ def main() -> int:
                 "rel_l2_AET": [float(np.linalg.norm(got[c, T:-T] - aet_ref[c, T:-T]) /
                                      np.linalg.norm(aet_ref[c, T:-T])) for c in range(3)],
                 "max_abs_AET": [float(np.abs(got[c] - aet_ref[c]).max()) for c in range(3)],
-                "bitwise_equal": bool(np.array_equal(got, aet_ref))}
+                "bitwise_equal": bool(np.array_equal(got, aet_ref)),
+                "rel_l2_AET_vs_wide": vs_wide(got)}
         line["parity"] = par
 
     if args.batch_sources > 0 and args.only is None:
@@ -517,6 +531,17 @@ def main() -> int:
         s = serving_run(args.serve_seconds, args.serve_rate)
         line["latency"] = {"p50_ms": s["p50_ms"], "p99_ms": s["p99_ms"]}
         line["serving"] = s
+        # the same traffic at lower arrival rates: latency below saturation
+        curve = []
+        for rate in (r for r in (0.5 * args.serve_rate, 0.75 * args.serve_rate) if r > 0):
+            c = serving_run(args.serve_seconds, rate)
+            curve.append({k: c[k] for k in ("arrival_rate_per_s", "p50_ms", "p99_ms",
+                                             "samples_per_s", "offered_samples_per_s",
+                                             "gpu_busy_frac", "queue_max")})
+        curve.append({k: s[k] for k in ("arrival_rate_per_s", "p50_ms", "p99_ms",
+                                        "samples_per_s", "offered_samples_per_s",
+                                        "gpu_busy_frac", "queue_max")})
+        line["latency_curve"] = curve
 
     if args.copy_probe > 0:
         src = torch.ones(args.copy_probe // 8, dtype=torch.float64, device="cuda")

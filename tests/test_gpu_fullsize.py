@@ -122,6 +122,45 @@ def test_year_fp64_dropin_bitwise(torch_gpu, orbits10, year_case):
     np.testing.assert_array_equal(got, aet)
 
 
+@pytest.fixture(scope="module")
+def year_wide(orbits10, oracle_orbits10, oracle_mod, year_case):
+    """The 1-yr source through the oracle built with extended-precision values (time, delay and
+    index arithmetic unchanged): the exact values the FP64 forms round."""
+    h, _ = year_case
+    _, aet = oracle_mod.response_pipeline(oracle_orbits10, h, 1.2, 0.3, 10.0, 25, "2nd generation",
+                                          x_base=orbits10.x_base, nthreads=NTH, wide=True)
+    return aet.astype(np.float64)
+
+
+def test_year_fast_fp64_forms_at_the_fp64_floor(torch_gpu, orbits10, year_case, year_wide):
+    """The division-free FP64 forms differ from the oracle by ~6e-12 on the null channel T. Both
+    are compared with the extended-precision result: the oracle (the reference's own FP64
+    arithmetic) is 5.6e-12 from it on T, and the fast forms are no farther on any channel --
+    their error is the FP64 rounding floor of this workload, not a defect of the form."""
+    torch = torch_gpu
+    from fastlisaresponse_amd import pyResponseTDI
+    from fastlisaresponse_amd.batch import BatchedResponse
+    h, aet = year_case
+    T = T_BUF
+    floor = [rel_l2(aet[c, T:-T], year_wide[c, T:-T]) for c in range(3)]
+    assert floor[0] < 1e-14 and floor[1] < 1e-14 and 1e-12 < floor[2] < 1e-11
+    hd = torch.as_tensor(h, device="cuda")
+    r = pyResponseTDI(0.1, N_YEAR, order=25, tdi="2nd generation", orbits=orbits10, tdi_chan="AET",
+                      exact=False)
+    r.get_projections(hd, 1.2, 0.3)
+    fast = torch.stack(r.get_tdi_delays()).cpu().numpy()
+    br = BatchedResponse(0.1, tdi="2nd generation", orbits=orbits10, tdi_chan="AET", fused=True,
+                         exact=False)
+    plan = br.plan(hd, [N_YEAR], [1.2], [0.3])
+    plan.run()
+    plan.check_errors()
+    fused = plan.out_all.view(3, -1).cpu().numpy()
+    for name, got in (("fast drop-in", fast), ("fused", fused)):
+        for c in range(3):
+            err = rel_l2(got[c, T:-T], year_wide[c, T:-T])
+            assert err <= 1.1 * floor[c], (name, c, err, floor[c])
+
+
 @pytest.mark.parametrize("precision,tols", [("mixed-c128", (1e-7, 1e-7, 1e-7)),
                                             ("mixed", (1e-6, 1e-6, 1e-3))])
 def test_year_mixed_vs_oracle(torch_gpu, orbits10, year_case, precision, tols):
