@@ -400,11 +400,26 @@ int make_tdi_config(const flr_response *r, double dt, double fs, int flags, TdiC
     return FLR_OK;
 }
 
+// The fused fast kernels take the order-25 E weights as compile-time constants (e25): true when
+// the response's uploaded table holds exactly those values.
+bool stock_e25(const flr_response *r)
+{
+    if (r->order != 25 || r->E_host.size() < 12) return false;
+    for (int j = 0; j < 12; ++j)
+        if (r->E_host[j] != e25(j)) return false;
+    return true;
+}
+
 // Fused launch: the TDI window of a tile of passes * kBlock samples, and the h window that
 // projects it.
 int make_fused_config(const flr_response *r, double dt, double fs, int flags, int passes,
                       ProjConfig &pc, TdiConfig &tc, StockTdi &stock)
 {
+    if (!(flags & FLR_FLAG_EXACT) && !stock_e25(r))
+        return fail(FLR_ERR_UNSUPPORTED,
+                    "the fast fused kernels are built for the order-25 Lagrange E table of "
+                    "pyResponseTDI._fill_A_E and the uploaded table differs (use FLR_FLAG_EXACT "
+                    "or the unfused entry points)");
     int st = make_proj_config(r, dt, fs, flags, pc);
     if (st) return st;
     if ((st = make_tdi_config(r, dt, fs, flags | FLR_FLAG_OVERWRITE, tc, stock))) return st;

@@ -70,6 +70,8 @@ __device__ __forceinline__ OrbitNode orbit_node(const OrbitView &o, double t)
 }
 
 // Column: 0..5 light travel time of table row r; 6 + 3*s + c position of spacecraft s (0..2).
+// Multiply then add, in every form: positions feed the arrival-time delays, which follow the
+// reference's operation sequence.
 __device__ __forceinline__ double orbit_val(const OrbitNode &n, int col)
 {
     return n.p[kOrbitVals + col] * n.tau + n.p[col];
@@ -108,6 +110,17 @@ __device__ __forceinline__ void split_delay(double delay, double t0_offset, doub
     e = (1.0 + (double)k) - cf;
 }
 
+// Reciprocal square root of a positive normal x: the hardware seed y (relative error t' below
+// 2^-22) refined by the binomial series (1 - t)^(-1/2) = 1 + t/2 + 3t^2/8 + ... with
+// t = 1 - x y^2, to third order in t -- full double precision, without the special-value
+// fix-ups of the library routine (the callers pass squared link lengths).
+__device__ __forceinline__ double rsqrt_pos(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    const double t = __builtin_fma(-x * y, y, 1.0);
+    return __builtin_fma(y * t, __builtin_fma(t, 0.375, 0.5), y);
+}
+
 // ------------------------------------------------------------------ Lagrange prefactor A(e)
 // Linear interpolation in the 1001-point table of A(e) (response.py:279-298), indexed like
 // LISAResponse.cu:114-117; y_deps = rcp_nr2(deps). The index is clamped so that no e can read
@@ -131,6 +144,49 @@ __device__ __forceinline__ double lagrange_A_fast(const LagrangeView &lg, double
     double frac = s - (double)ind;
     double2 a = lg.Aslope[ind];
     return __builtin_fma(a.y, frac, a.x);
+}
+
+// The order-25 (h = 13) weights E[j-1] = (-1)^j (h-1)!/(h-1-j)! h!/(h+j)!, j = 1..12, with
+// the factorials rounded to double first, as pyResponseTDI._fill_A_E builds them
+// (response.py:300-310): the exact bits of the table the host uploads. Kernels compiled for
+// order 25 take them as constants (no kernel-argument registers); the host launches them only
+// when the uploaded table holds these values (stock_e25).
+__host__ __device__ constexpr double e25(int jm1)
+{
+    return jm1 == 0 ? -0x1.b6db6db6db6dbp-1 : jm1 == 1 ? 0x1.41d41d41d41d5p-1
+         : jm1 == 2 ? -0x1.924924924924ap-2 : jm1 == 3 ? 0x1.a9f317a9f317ap-3
+         : jm1 == 4 ? -0x1.7a9f317a9f318p-4 : jm1 == 5 ? 0x1.16fc0982c6248p-5
+         : jm1 == 6 ? -0x1.4ec80b69ba923p-7 : jm1 == 7 ? 0x1.3ed6e64c5029bp-9
+         : jm1 == 8 ? -0x1.cfc437b4d1b0fp-12 : jm1 == 9 ? 0x1.e3ee23de1275ep-15
+         : jm1 == 10 ? -0x1.429ec2940c4eap-18 : 0x1.9cf42c429f1cep-23;
+}
+
+// E[j-1]: the compile-time order-25 table (CE) or the uploaded one.
+template <bool CE>
+__device__ __forceinline__ double lagrange_E(const LagrangeView &lg, int jm1)
+{
+    if constexpr (CE)
+        return e25(jm1);
+    else
+        return lg.E[jm1];
+}
+
+// Where a stencil reads E[j-1]: the uploaded table in the kernel arguments (registers), the
+// compile-time order-25 table, or a copy of the uploaded table in LDS (read per use, keeping
+// the table out of the scalar registers of long loops).
+constexpr int kEArg = 0;
+constexpr int kEConst = 1;
+constexpr int kELds = 2;
+
+template <int ESRC>
+__device__ __forceinline__ double stencil_E(const LagrangeView &lg, const double *E_lds, int jm1)
+{
+    if constexpr (ESRC == kEConst)
+        return e25(jm1);
+    else if constexpr (ESRC == kELds)
+        return E_lds[jm1];
+    else
+        return lg.E[jm1];
 }
 
 // Explicit weight of tap x[d + r], r in [-(h-1), h], in the fast stencil at fraction e (the
@@ -275,6 +331,49 @@ __device__ __forceinline__ double2 stencil_fast_c(const double2 *x, double e, co
     }
     double invP = rcp_nr2(P);
     double2 x1 = x[1], x0 = x[0];
+    double2 out;
+    out.x = A * __builtin_fma(D, Tr * invP, __builtin_fma(B, x1.x, e * x0.x));
+    out.y = A * __builtin_fma(D, Ti * invP, __builtin_fma(B, x1.y, e * x0.y));
+    return out;
+}
+
+// The fast complex stencil addressed from its lowest tap, lo = x - (H-1): every tap sits at a
+// non-negative constant offset from one base address (x[d+1+j] = lo[H+j], x[d-j] = lo[H-1-j]),
+// so no per-tap address arithmetic is needed. Same operation sequence as stencil_fast_c; E from
+// where ESRC says (stencil_E).
+template <int H, int ESRC>
+__device__ __forceinline__ double2 stencil_fast_c_lo(const double2 *lo, double e,
+                                                     const LagrangeView &lg, const double *E_lds)
+{
+    static_assert(H > 0, "compile-time half point count");
+    double A = lagrange_A_fast(lg, e);
+    double B = 1.0 - e;
+    double D = e * B;
+    double Tr = 0.0, Ti = 0.0;
+    double P = 1.0;
+#pragma unroll
+    for (int j = 1; j < H; j += 1) {
+        double2 up = lo[H + j];
+        double2 down = lo[H - 1 - j];
+        double Q = (double)(j * (j + 1)) + D;
+        double EP = stencil_E<ESRC>(lg, E_lds, j - 1) * P;
+        double eEP = e * EP;
+        double wu = __builtin_fma((double)(j + 1), EP, -eEP);
+        double wd = __builtin_fma((double)j, EP, eEP);
+        const double nr = __builtin_fma(up.x, wu, down.x * wd);
+        const double ni = __builtin_fma(up.y, wu, down.y * wd);
+        if (j == 1) {
+            Tr = nr;
+            Ti = ni;
+        } else {
+            Tr = __builtin_fma(Tr, Q, nr);
+            Ti = __builtin_fma(Ti, Q, ni);
+        }
+        P = P * Q;
+        __asm__ volatile("" ::: "memory");  // one tap pair in registers at a time
+    }
+    double invP = rcp_nr2(P);
+    double2 x1 = lo[H], x0 = lo[H - 1];
     double2 out;
     out.x = A * __builtin_fma(D, Tr * invP, __builtin_fma(B, x1.x, e * x0.x));
     out.y = A * __builtin_fma(D, Ti * invP, __builtin_fma(B, x1.y, e * x0.y));

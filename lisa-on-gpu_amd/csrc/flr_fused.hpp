@@ -26,6 +26,13 @@ constexpr int kModeGroups = 1;
 constexpr int kModeLinear = 2;
 constexpr int kModeStock = 3;  // stock table, unequal arms: tdi_stock_unequal
 
+// Copy the E table into LDS (read back by the fast stencils after the next barrier).
+__device__ __forceinline__ void load_E_lds(const LagrangeView &lg, double *s_E)
+{
+    const int x = threadIdx.x;
+    if (x < kMaxHalf) s_E[x] = lg.E[x];
+}
+
 template <typename HT>
 __device__ __forceinline__ double2 load_h(const void *h, int j)
 {
@@ -184,7 +191,7 @@ template <bool EXACT, int MODE, int NX, typename HT, typename OT, int PASSES, in
 __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig &tc,
                                            const FusedSource &src, int s, int tile,
                                            const OrbitView &orb, const LagrangeView &lg,
-                                           const RowMap &rm, double *smem)
+                                           const RowMap &rm, double *smem, const double *s_E)
 {
     constexpr int H = 13;
     constexpr int kPasses = PASSES;
@@ -247,13 +254,14 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
             const double t = src.t0 + (double)j * pc.dt;
             // linear mode: the link rows are only read back by linear_series just below, so they
             // are kept in canonical link order (compile-time rows)
+            // fast forms: E read from the workgroup's LDS copy (s_E), not kept in registers
             bad |= project_links<EXACT, H, MODE == kModeGroups || MODE == kModeLinear,
-                                 MODE == kModeLinear>(pc, src.k, src.u, src.v, t, t0_offset, orb, lg,
-                                                      hwin, hw_lo, hw_len, y_deps,
-                                                      [&](int row, double val) { yc[row * pitch] = val; });
+                                 MODE == kModeLinear, EXACT ? kEArg : kELds>(
+                pc, src.k, src.u, src.v, t, t0_offset, orb, lg, hwin, hw_lo, hw_len, y_deps, s_E,
+                [&](int row, double val) { yc[row * pitch] = val; });
         }
         if constexpr (MODE == kModeLinear)
-            linear_series<true>(yc, pitch, rm, tc.aet, tc.s2, tc.s6, tc.s3);
+            linear_series<true>(yc, pitch, rm, tc.aet);
     }
     __syncthreads();
 
@@ -289,8 +297,8 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
             const bool ok = lti && lti_matches<NX>(t, i, L, t0_offset, tc.fs, lref, yw_lo, yw_len);
             lti_ok[i - i0] = ok;
             if (ok) continue;
-            bad |= !tdi_linear_padded<NX>(t, L, t0_offset, tc.fs, ywin, pitch, yw_lo, yw_len, lg,
-                                          acc);
+            bad |= !tdi_linear_padded<NX, true>(t, L, t0_offset, tc.fs, ywin, pitch, yw_lo,
+                                                yw_len, lg, acc);
             fused_store<OT>(tc, src, i, acc, false);
             continue;
         }
@@ -321,7 +329,8 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
                 bad |= !tdi_equal_arm<NX, EXACT>(t, L, t0_offset, tc.fs, ywin, rowoff, yw_lo,
                                                  yw_len, lg, y_deps, acc);
             } else {
-                bad |= !tdi_linear<NX>(t, L, t0_offset, tc.fs, ywin, cap, yw_lo, yw_len, lg, acc);
+                bad |= !tdi_linear<NX, true>(t, L, t0_offset, tc.fs, ywin, cap, yw_lo, yw_len, lg,
+                                             acc);
             }
         }
         fused_store<OT>(tc, src, i, acc, MODE != kModeLinear && tc.aet);
@@ -344,7 +353,9 @@ void fused_batch_kernel(ProjConfig pc, TdiConfig tc,
                                                              RowMap rm)
 {
     extern __shared__ double smem[];
-    FLR_BATCH_LOOP(batch, (fused_tile<EXACT, MODE, NX, HT, OT, PASSES>(pc, tc, src, s_, tile, orb, lg, rm, smem)));
+    __shared__ double s_E[kMaxHalf];  // the E table, read by the fast stencils
+    load_E_lds(lg, s_E);  // visible after the loop's first barrier
+    FLR_BATCH_LOOP(batch, (fused_tile<EXACT, MODE, NX, HT, OT, PASSES>(pc, tc, src, s_, tile, orb, lg, rm, smem, s_E)));
 }
 
 // The mixed equal-arm form (composite filters on the matrix cores), held to 4 waves per SIMD:
@@ -357,7 +368,9 @@ void fused_lti_kernel(ProjConfig pc, TdiConfig tc, Batch<FusedSource> batch, Orb
                       LagrangeView lg, RowMap rm)
 {
     extern __shared__ double smem[];
-    FLR_BATCH_LOOP(batch, (fused_tile<false, kModeLinear, NX, HT, float, PASSES, TPB>(pc, tc, src, s_, tile, orb, lg, rm, smem)));
+    __shared__ double s_E[kMaxHalf];
+    load_E_lds(lg, s_E);
+    FLR_BATCH_LOOP(batch, (fused_tile<false, kModeLinear, NX, HT, float, PASSES, TPB>(pc, tc, src, s_, tile, orb, lg, rm, smem, s_E)));
 }
 
 // Launch one fused variant over a batch: the TDI mode follows the table (stock + equal arms:

@@ -34,17 +34,29 @@ __device__ __forceinline__ TileRange tile_range(int tile, int num_delays, int st
 // geometry and stencil; store(row, value) receives y of table row `row` (CANON: of the link's
 // canonical index stock::canon, a compile-time row). Operation order per link follows
 // LISAResponse.cu:559-668. Returns true if a stencil fell outside the window (its links get
-// NaN).
-template <bool EXACT, int H, bool EQ, bool CANON = false, typename Store>
+// NaN). ESRC: where the fast stencils read E (stencil_E; E_lds for kELds).
+template <bool EXACT, int H, bool EQ, bool CANON = false, int ESRC = kEArg, typename Store>
 __device__ __forceinline__ bool project_links(const ProjConfig &cfg, const double (&k)[3],
                                               const double (&u)[3], const double (&v)[3],
                                               double t, double t0_offset, const OrbitView &orb,
                                               const LagrangeView &lg, const double2 *hwin,
-                                              int win_lo, int win_len, double y_deps, Store &&store)
+                                              int win_lo, int win_len, double y_deps,
+                                              const double *E_lds, Store &&store)
 {
     const int h = H > 0 ? H : lg.h;
     const OrbitNode nr = orbit_node<EXACT>(orb, t);
     bool bad = false;
+    // one complex stencil at window offset l (sample d = win_lo + l) and fraction e; the fast
+    // form with a compile-time order reads every tap at a constant offset from the lowest one
+    auto cstencil = [&](int l, double e) -> double2 {
+        if constexpr (!EXACT && H > 0) {
+            int lo = l - (H - 1);
+            pin(lo);  // an opaque base: the taps are not re-addressed from another register
+            return stencil_fast_c_lo<H, ESRC>(hwin + lo, e, lg, E_lds);
+        } else {
+            return stencil_c<EXACT, H>(hwin + l, e, lg, y_deps);
+        }
+    };
     // EQ (all six light travel times bitwise equal): the emission time t - L, the emitter's
     // position and so the emitter stencil depend on the emitting spacecraft only -- 3 emitter
     // stencils instead of 6, each bit-identical to the per-link one.
@@ -64,7 +76,7 @@ __device__ __forceinline__ bool project_links(const ProjConfig &cfg, const doubl
             const int le = d_em - win_lo;
             bem[se] = (le - (h - 1) < 0) || (le + h >= win_len);
             hem[se] = make_double2(0.0, 0.0);
-            if (!bem[se]) hem[se] = stencil_c<EXACT, H>(hwin + le, e_em, lg, y_deps);
+            if (!bem[se]) hem[se] = cstencil(le, e_em);
         });
     }
     static_for<3>([&](auto s) {
@@ -77,7 +89,7 @@ __device__ __forceinline__ bool project_links(const ProjConfig &cfg, const doubl
         const int lr = d_rec - win_lo;
         const bool br = (lr - (h - 1) < 0) || (lr + h >= win_len);
         double2 hr = make_double2(0.0, 0.0);
-        if (!br) hr = stencil_c<EXACT, H>(hwin + lr, e_rec, lg, y_deps);
+        if (!br) hr = cstencil(lr, e_rec);
         static_for<2>([&](auto q) {
             constexpr int se = (s + 1 + q) % 3;  // the emitters of receiver s
             const int row = CANON ? stock::canon(10 * (s + 1) + se + 1) : cfg.link_row[s][se];
@@ -99,7 +111,7 @@ __device__ __forceinline__ bool project_links(const ProjConfig &cfg, const doubl
                 split_delay(t_em - dot3(k, xe) * kCInv, t0_offset, cfg.fs, d_em, e_em);
                 const int le = d_em - win_lo;
                 bq = br || (le - (h - 1) < 0) || (le + h >= win_len);
-                if (!bq) he = stencil_c<EXACT, H>(hwin + le, e_em, lg, y_deps);
+                if (!bq) he = cstencil(le, e_em);
             }
             double norm = 0.0;
             if constexpr (EXACT) {
@@ -119,7 +131,7 @@ __device__ __forceinline__ bool project_links(const ProjConfig &cfg, const doubl
 #pragma unroll
                 for (int c = 0; c < 3; ++c) n[c] = n[c] / norm;
             } else {  // one reciprocal square root instead of a root and three divisions
-                const double inv = rsqrt(norm);
+                const double inv = rsqrt_pos(norm);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) n[c] = n[c] * inv;
             }

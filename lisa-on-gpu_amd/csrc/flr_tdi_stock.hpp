@@ -85,8 +85,9 @@ constexpr int group_sign(int m)
 {
     return pair_sign(m, 13, 12) != 0 ? pair_sign(m, 13, 12) : pair_sign(m, 31, 21);
 }
-static_assert(group_sign(0) && group_sign(1) && group_sign(2) && group_sign(3) &&
-                  group_sign(4) && group_sign(5) && group_sign(6) && group_sign(7),
+static_assert(group_sign(0) != 0 && group_sign(1) != 0 && group_sign(2) != 0 &&
+                  group_sign(3) != 0 && group_sign(4) != 0 && group_sign(5) != 0 &&
+                  group_sign(6) != 0 && group_sign(7) != 0,
               "every chain length pairs two links of one kind");
 
 
@@ -283,13 +284,18 @@ __device__ __forceinline__ bool tdi_stock_unequal(double t, const double (&L)[6]
     return ok;
 }
 
+// 1/sqrt(2), 1/sqrt(6), 1/sqrt(3) rounded to double: the combined series form A, E, T with
+// one multiplication each (the fast forms; within an ulp of the division by the rounded root).
+constexpr double kInvSqrt2 = 0.70710678118654752440084436210484903928;
+constexpr double kInvSqrt6 = 0.40824829046386301636621401245098189866;
+constexpr double kInvSqrt3 = 0.57735026918962576450914878050195745565;
+
 // Column of the combined series (in place of the six y rows, same stride `cap`): row
 // 2*o + kind holds output o's combination of kind `kind` -- X, Y, Z themselves, or A, E, T
 // formed from them when aet is set. yc points at the column; rm maps canonical links to rows
 // (CANON: the rows are already in canonical order).
 template <bool CANON = false>
-__device__ __forceinline__ void linear_series(double *yc, int cap, const RowMap &rm, bool aet,
-                                              double s2, double s6, double s3)
+__device__ __forceinline__ void linear_series(double *yc, int cap, const RowMap &rm, bool aet)
 {
     double y[6];
 #pragma unroll
@@ -303,9 +309,9 @@ __device__ __forceinline__ void linear_series(double *yc, int cap, const RowMap 
     for (int kind = 0; kind < 2; ++kind) {
         const double X = kd[kind][0], Y = kd[kind][1], Z = kd[kind][2];
         if (aet) {
-            yc[(0 + kind) * cap] = (Z - X) / s2;
-            yc[(2 + kind) * cap] = (X - 2.0 * Y + Z) / s6;
-            yc[(4 + kind) * cap] = (X + Y + Z) / s3;
+            yc[(0 + kind) * cap] = (Z - X) * kInvSqrt2;
+            yc[(2 + kind) * cap] = (X - 2.0 * Y + Z) * kInvSqrt6;
+            yc[(4 + kind) * cap] = (X + Y + Z) * kInvSqrt3;
         } else {
             yc[(0 + kind) * cap] = X;
             yc[(2 + kind) * cap] = Y;
@@ -320,7 +326,7 @@ __device__ __forceinline__ void linear_series(double *yc, int cap, const RowMap 
 // fell outside the window (the outputs are NaN). The taps are read through
 // cols(o, kind, c0) -> a pointer p with p[k] = window column c0 + k of output o's series of
 // that kind (k = 0 .. 2H-1).
-template <int NX, typename Cols>
+template <int NX, bool CE = false, typename Cols>
 __device__ __forceinline__ bool tdi_linear_cols(double t, double L, double t0_offset, double fs,
                                                 int win_lo, int win_len, const LagrangeView &lg,
                                                 Cols cols, double (&acc)[3])
@@ -347,7 +353,7 @@ __device__ __forceinline__ bool tdi_linear_cols(double t, double L, double t0_of
 #pragma unroll
         for (int j = 1; j < H; ++j) {
             const double Q = (double)(j * (j + 1)) + D;
-            const double EP = lg.E[j - 1] * P;
+            const double EP = lagrange_E<CE>(lg, j - 1) * P;
             const double eEP = e * EP;
             // the pair's weights times E_{j-1} P_{j-1}, shared by the three outputs
             const double wu = __builtin_fma((double)(j + 1), EP, -eEP);
@@ -380,12 +386,12 @@ __device__ __forceinline__ bool tdi_linear_cols(double t, double L, double t0_of
 
 // Contiguous rows of pitch cap: every tap at a non-negative immediate offset from one pinned
 // base address per output.
-template <int NX>
+template <int NX, bool CE = false>
 __device__ __forceinline__ bool tdi_linear(double t, double L, double t0_offset, double fs,
                                            const double *ywin, int cap, int win_lo, int win_len,
                                            const LagrangeView &lg, double (&acc)[3])
 {
-    return tdi_linear_cols<NX>(t, L, t0_offset, fs, win_lo, win_len, lg,
+    return tdi_linear_cols<NX, CE>(t, L, t0_offset, fs, win_lo, win_len, lg,
                                [&](int o, int kind, int c0) {
                                    int off = (2 * o + kind) * cap + c0;
                                    pin(off);
@@ -408,13 +414,13 @@ struct PaddedTaps {
 };
 
 // tdi_linear on padded rows (the per-sample fallback of the matrix-core path).
-template <int NX>
+template <int NX, bool CE = false>
 __device__ __forceinline__ bool tdi_linear_padded(double t, double L, double t0_offset, double fs,
                                                   const double *ywin, int pitch, int win_lo,
                                                   int win_len, const LagrangeView &lg,
                                                   double (&acc)[3])
 {
-    return tdi_linear_cols<NX>(t, L, t0_offset, fs, win_lo, win_len, lg,
+    return tdi_linear_cols<NX, CE>(t, L, t0_offset, fs, win_lo, win_len, lg,
                                [&](int o, int kind, int c0) {
                                    return PaddedTaps{ywin + (2 * o + kind) * pitch, c0};
                                },

@@ -1,5 +1,7 @@
 """Host-side behaviour of the Python layer that needs no GPU: argument validation (errors the
 reference raises before any kernel runs), orbit providers and their table layout."""
+import os
+
 import numpy as np
 import pytest
 
@@ -206,3 +208,18 @@ def test_serving_admission_respects_budget_and_head_of_line():
         srv.pending.append(r)
     srv.admit()
     assert srv.launched == [(0, [0])]
+
+
+def test_compile_time_order25_E_table_matches_the_host_table():
+    """The fused fast kernels' compile-time order-25 E weights (e25 in flr_device.hpp) are the
+    bits of the table pyResponseTDI uploads (the host refuses other tables for those kernels)."""
+    import re
+    from conftest import ROOT
+    from fastlisaresponse_amd.tables import lagrange_tables
+    text = open(os.path.join(ROOT, "lisa-on-gpu_amd", "csrc", "flr_device.hpp")).read()
+    body = text[text.index("constexpr double e25(int jm1)"):]
+    body = body[:body.index("}")]
+    consts = [float.fromhex(h) for h in re.findall(r"(-?0x1\.[0-9a-f]+p-?\d+)", body)]
+    _, _, E = lagrange_tables(25)
+    assert len(consts) == 12
+    assert consts == [float(e) for e in E[:12]]
