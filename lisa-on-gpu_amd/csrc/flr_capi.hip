@@ -90,6 +90,8 @@ struct flr_orbits {
     double max_radius = 0.0;
     double max_ltt = 0.0;
     bool equal_arm = false;  // all six light-travel-time columns bitwise equal at every node
+    bool constant_arm = false;  // ... and equal at every node (zero slopes): ltt0 everywhere
+    double ltt0 = 0.0;
     int device = 0;
 
     int row_of(int link) const
@@ -169,6 +171,9 @@ struct flr_batch {
     bool ran_recorded = false;       // `ran` has been recorded at least once
     double *d_lti = nullptr;         // mixed fused: [capacity][kLtiRecord] composite TDI filters
     uint64_t lti_version = ~0ull;    // response version the filters were built for (none yet)
+    bool lti_uniform = false;        // every bound source clears the composite-filter tile test
+    uint64_t uniform_version = ~0ull;  // response version and tile length lti_uniform holds for
+    int uniform_tile = 0;
 };
 
 namespace {
@@ -410,6 +415,45 @@ bool stock_e25(const flr_response *r)
     return true;
 }
 
+// Host replica of the composite-filter path's decisions for one source (flr_fused.hpp:
+// lti_source_reference, lti_reference, lti_tile_matches), in the same FP64 operations: true when
+// the source's filters fit and every one of its tiles clears the tile test as a whole. The test
+// of a tile is monotone in its last sample and the orbit nodes it covers, so the whole TDI range
+// [T, n - T - 1] decides for every tile. Needs constant, equal arms (the only case where the
+// composite filters are used throughout).
+bool lti_source_uniform(const flr_orbits *o, const FusedSource &f, double dt, double fs, int nx,
+                        int halo_lo, int halo_hi)
+{
+    if (!o->constant_arm) return false;
+    const int n = f.num_pts, T = f.tdi_start;
+    if (n - 2 * T <= 0) return true;  // no sample reaches the TDI stage
+    const int M = nx == 8 ? 3 : 7;    // longest chain
+    const int H = 13;
+    const double L_ref = o->ltt0;
+    // lti_reference at the source's first TDI sample
+    const double t_ref = f.t0 + (double)T * dt;
+    double Dm = t_ref;
+    int lo = 1 << 30, hi = -(1 << 30);
+    for (int m = 0; m <= M; ++m) {
+        if (m > 0) Dm -= L_ref;
+        const double clipped = Dm - f.t0;
+        const double cf = clipped * fs;
+        const int d = (int)std::ceil(cf) - 1;
+        lo = std::min(lo, d - T);
+        hi = std::max(hi, d - T);
+    }
+    const int kmin = lo - (H - 1);
+    if (!((hi + H) - kmin + 1 <= kFir)) return false;
+    // lti_tile_matches over the whole range
+    if (kmin + halo_lo < 0 || kmin + kFir > halo_hi + 1) return false;
+    const int last = n - T - 1;
+    auto ulp = [](double x) { return std::ldexp(1.0, std::ilogb(x) - 52); };
+    const double tmax = 1.001 * (2.0 * std::fabs(f.t0) + (double)last * dt + M * std::fabs(L_ref));
+    const double per_sample = (0.5 * M + 1.5) * ulp(tmax) * fs + ulp(tmax * fs);
+    const double drift = 1.01 * std::fabs(std::fma(dt, fs, -1.0)) * (double)(last - T);
+    return 2.0 * per_sample + drift + 1e-13 <= kLtiTol;
+}
+
 // Fused launch: the TDI window of a tile of passes * kBlock samples, and the h window that
 // projects it.
 int make_fused_config(const flr_response *r, double dt, double fs, int flags, int passes,
@@ -515,6 +559,9 @@ int check_errors_impl(flr_response *r, hipStream_t stream)
         return fail(FLR_ERR_WINDOW,
                     "a stencil reached outside the staged sample window: the start index is too "
                     "small for the delays, or the time grid is not uniform");
+    if (word & kErrLti)
+        return fail(FLR_ERR_HIP, "internal: a composite-filter tile failed the test the host had "
+                                 "proved it passes (its outputs are NaN)");
     return FLR_OK;
 }
 
@@ -608,6 +655,10 @@ int flr_orbits_create(double dt, int N, const double *n_arr, const double *ltt_a
     for (int w = 0; w < N && o->equal_arm; ++w)
         for (int r = 1; r < kNLinks; ++r)
             if (table[(size_t)w * kOrbitStride + r] != table[(size_t)w * kOrbitStride]) o->equal_arm = false;
+    o->ltt0 = table[0];
+    o->constant_arm = o->equal_arm;
+    for (int w = 1; w < N && o->constant_arm; ++w)
+        if (table[(size_t)w * kOrbitStride] != o->ltt0) o->constant_arm = false;
     hipError_t e = hipGetDevice(&o->device);
     if (e == hipSuccess) e = hipMalloc(&o->d_table, table.size() * sizeof(double));
     if (e == hipSuccess)
@@ -1082,6 +1133,7 @@ int flr_fused_bind(flr_batch *b, const flr_source *sources, int num_sources, con
     b->samples = samples;
     b->fused_tiles = ft[num_sources];
     if (!same_refs) b->lti_version = ~0ull;  // new sources: their filters are built now
+    b->uniform_version = ~0ull;               // and the tile-test proof is redone at the run
     const int fst = lti_refresh(b, st);
     // recorded after the filters too: flr_fused_create waits on it before returning
     FLR_HIP(hipEventRecord(b->uploaded, st));
@@ -1130,6 +1182,22 @@ int flr_batch_run(flr_batch *b, void *stream)
         if ((st = lti_refresh(b, s))) return st;
         Batch<FusedSource> bf{b->d_fused, b->d_fused_tiles, b->num_sources, b->fused_tiles};
         tc.lti_fir = b->d_lti;
+        if ((b->flags & FLR_FLAG_MIXED) && stock.generation != 0 && stock.equal_arm &&
+            r->order == 25) {
+            if (b->uniform_version != r->version || b->uniform_tile != tc.tile) {
+                // the staging copy holds the bound sources (its upload may still be in flight:
+                // reading it on the host is fine)
+                bool all = true;
+                for (int q = 0; q < b->num_sources && all; ++q)
+                    all = lti_source_uniform(r->orbits, b->h_fused[q], b->dt, b->fs,
+                                             stock.generation == 1 ? 8 : 16, tc.halo_lo,
+                                             tc.halo_hi);
+                b->lti_uniform = all;
+                b->uniform_version = r->version;
+                b->uniform_tile = tc.tile;
+            }
+            tc.lti_uniform = b->lti_uniform ? 1 : 0;
+        }
         hipError_t e = launch_fused_batch(pc, tc, bf, r->orbits->view(), r->lagrange(), exact,
                                           (b->flags & FLR_FLAG_MIXED) != 0,
                                           (b->flags & FLR_FLAG_Q31) ? kInQ31
@@ -1365,6 +1433,16 @@ int flr_fused_run_static(flr_response *r, const flr_source *sources, int num_sou
     const OrbitView orb = r->orbits->view();
     if (mixed) {
         tc.lti_fir = d_lti;
+        if (stock.generation != 0 && stock.equal_arm && r->order == 25) {
+            bool all = true;
+            for (int q = 0; q < num_sources && all; ++q) {
+                FusedSource f;
+                (void)fused_source_record(sources[q], q, flags, h_all, out_all, f);
+                all = lti_source_uniform(r->orbits, f, dt, sampling_frequency,
+                                         stock.generation == 1 ? 8 : 16, tc.halo_lo, tc.halo_hi);
+            }
+            tc.lti_uniform = all ? 1 : 0;
+        }
         hipError_t e = launch_lti_filters(tc, bf, orb, lg, stock, s);
         if (e != hipSuccess) return hip_fail(e, "composite filter launch");
     }

@@ -187,7 +187,12 @@ __device__ __forceinline__ void lti_mfma(const FusedSource &src, int first, int 
     }
 }
 
-template <bool EXACT, int MODE, int NX, typename HT, typename OT, int PASSES, int TPB = kBlock>
+// SAMPLE_PASS (composite-filter path): the per-sample test and fallback stencils for tiles that
+// cannot be cleared as a whole are compiled in. Without them (when the host has proved that
+// every tile clears, tc.lti_uniform) the kernel is smaller and its projection loop keeps its
+// operands in registers; a tile that fails anyway is reported (kErrLti) and written as NaN.
+template <bool EXACT, int MODE, int NX, typename HT, typename OT, int PASSES, int TPB = kBlock,
+          bool SAMPLE_PASS = true>
 __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig &tc,
                                            const FusedSource &src, int s, int tile,
                                            const OrbitView &orb, const LagrangeView &lg,
@@ -275,7 +280,9 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
         lref.kmin = (int)fir[kLtiKmin];
         lti = fir[kLtiOk] != 0.0;
         tile_ok = lti && lti_tile_matches<NX>(tc, src, orb, rm, lref.kmin, L_ref, first, last);
-        if (lti && !tile_ok) (void)lti_source_reference<NX, EXACT>(tc, src, orb, rm, lref);
+        if constexpr (SAMPLE_PASS) {
+            if (lti && !tile_ok) (void)lti_source_reference<NX, EXACT>(tc, src, orb, rm, lref);
+        }
         if (lti) {  // read by lti_mfma after the barrier that follows the per-sample pass
             for (int x = tid; x < 2 * kFpad; x += TPB) {
                 const int kind = x / kFpad, k = x - kind * kFpad - kFpadLo;
@@ -284,22 +291,38 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
         }
     }
 
+    if constexpr (kLti && !SAMPLE_PASS) {
+        if (!tile_ok) {  // cannot happen when the host's proof holds: report it loudly
+            for (int r = 0; r < kPasses; ++r) {
+                const int i = i0 + r * TPB + tid;
+                if (i < first || i > last) continue;
+                for (int c = 0; c < 3; ++c)
+                    store_out<OT>(src.out, (size_t)c * n + i, __builtin_nan(""));
+            }
+            if (tid == 0) atomicOr(tc.err, kErrLti);
+            lti = false;  // no matrix-core pass either
+        }
+    }
 #pragma unroll 1
-    for (int r = 0; r < (tile_ok ? 0 : kPasses); ++r) {  // tile_ok: every sample goes below
+    for (int r = 0; r < ((tile_ok || (kLti && !SAMPLE_PASS)) ? 0 : kPasses); ++r) {
+        // tile_ok: every sample goes to the matrix-core pass below
         const int i = i0 + r * TPB + tid;
         if (i < first || i > last) continue;
         const double t = src.t0 + (double)i * tc.dt;
         double acc[3] = {0.0, 0.0, 0.0};
         if constexpr (kLti) {
-            // samples whose delays match the reference are left to the matrix-core pass below;
-            // the others (if any) take the per-sample stencils here
-            const double L = orbit_val(orbit_node<EXACT>(orb, t), rm.row[0]);
-            const bool ok = lti && lti_matches<NX>(t, i, L, t0_offset, tc.fs, lref, yw_lo, yw_len);
-            lti_ok[i - i0] = ok;
-            if (ok) continue;
-            bad |= !tdi_linear_padded<NX, true>(t, L, t0_offset, tc.fs, ywin, pitch, yw_lo,
-                                                yw_len, lg, acc);
-            fused_store<OT>(tc, src, i, acc, false);
+            if constexpr (SAMPLE_PASS) {
+                // samples whose delays match the reference are left to the matrix-core pass
+                // below; the others (if any) take the per-sample stencils here
+                const double L = orbit_val(orbit_node<EXACT>(orb, t), rm.row[0]);
+                const bool ok =
+                    lti && lti_matches<NX>(t, i, L, t0_offset, tc.fs, lref, yw_lo, yw_len);
+                lti_ok[i - i0] = ok;
+                if (ok) continue;
+                bad |= !tdi_linear_padded<NX, true>(t, L, t0_offset, tc.fs, ywin, pitch, yw_lo,
+                                                    yw_len, lg, acc);
+                fused_store<OT>(tc, src, i, acc, false);
+            }
             continue;
         }
         if constexpr (MODE == kModeGeneric) {
@@ -362,7 +385,7 @@ void fused_batch_kernel(ProjConfig pc, TdiConfig tc,
 // its VGPRs and the matrix accumulator share one 512-entry register file per lane.
 // TPB threads per workgroup: 256, or 512 for the 1024-sample window (two rounds of 512 lanes;
 // its 72 KB of LDS then still leave 16 waves per CU, with half the halo recompute of 512).
-template <int NX, typename HT, int PASSES, int TPB>
+template <int NX, typename HT, int PASSES, int TPB, bool SAMPLE_PASS>
 __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4)))
 void fused_lti_kernel(ProjConfig pc, TdiConfig tc, Batch<FusedSource> batch, OrbitView orb,
                       LagrangeView lg, RowMap rm)
@@ -370,7 +393,7 @@ void fused_lti_kernel(ProjConfig pc, TdiConfig tc, Batch<FusedSource> batch, Orb
     extern __shared__ double smem[];
     __shared__ double s_E[kMaxHalf];
     load_E_lds(lg, s_E);
-    FLR_BATCH_LOOP(batch, (fused_tile<false, kModeLinear, NX, HT, float, PASSES, TPB>(pc, tc, src, s_, tile, orb, lg, rm, smem, s_E)));
+    FLR_BATCH_LOOP(batch, (fused_tile<false, kModeLinear, NX, HT, float, PASSES, TPB, SAMPLE_PASS>(pc, tc, src, s_, tile, orb, lg, rm, smem, s_E)));
 }
 
 // Launch one fused variant over a batch: the TDI mode follows the table (stock + equal arms:
@@ -394,9 +417,12 @@ hipError_t launch_fused_variant(const ProjConfig &pc, const TdiConfig &tc, const
             // the 1024-sample window runs as two rounds of 512 lanes
             constexpr int TPB = PASSES == 4 ? 2 * kBlock : kBlock;
             constexpr int P = PASSES == 4 ? 2 : PASSES;
-            return st.generation == 1
-                       ? launch_tpb(fused_lti_kernel<8, HT, P, TPB>, nb, TPB, shm, stream, pc, tc, b, orb, lg, rm)
-                       : launch_tpb(fused_lti_kernel<16, HT, P, TPB>, nb, TPB, shm, stream, pc, tc, b, orb, lg, rm);
+            // without the per-sample pass when the host proved every tile clears as a whole
+#define FLR_LTI(NXV, SP) \
+    launch_tpb(fused_lti_kernel<NXV, HT, P, TPB, SP>, nb, TPB, shm, stream, pc, tc, b, orb, lg, rm)
+            if (st.generation == 1) return tc.lti_uniform ? FLR_LTI(8, false) : FLR_LTI(8, true);
+            return tc.lti_uniform ? FLR_LTI(16, false) : FLR_LTI(16, true);
+#undef FLR_LTI
         } else {
             return st.generation == 1 ? FLR_FUSED(kEq, 8) : FLR_FUSED(kEq, 16);
         }

@@ -22,6 +22,7 @@ constexpr int kOrbitStride = 2 * kOrbitVals;
 
 // Device error word bits.
 constexpr int kErrWindow = 1;     // stencil outside the staged window (or the input array)
+constexpr int kErrLti = 2;        // a composite-filter tile failed a check the host had proved
 
 struct OrbitView {
     const double *table;  // [N][kOrbitStride]
@@ -89,6 +90,9 @@ struct TdiConfig {
     int tile;               // fused launches: output samples per tile
     double *lti_fir = nullptr;  // mixed fused launches: per source, the two composite TDI filters
                                 // ([2][kFir] doubles, written by lti_filter_kernel before the launch)
+    int lti_uniform = 0;        // mixed fused launches: the host proved that every tile of every
+                                // source clears the composite-filter test as a whole, so the
+                                // kernel without the per-sample pass runs
 };
 
 struct TdiSource {

@@ -615,3 +615,39 @@ def test_fused_mixed_graph_replays_across_rebind(torch_gpu, orbits, precision):
         assert torch.equal(plan.out_all, fresh.out_all), rep
     assert not torch.equal(eager, fresh.out_all)
     plan.check_errors()
+
+
+def test_lti_kernel_variants_agree(torch_gpu, oracle_mod):
+    """The composite-filter kernel comes in two variants: with the per-sample pass (some tile
+    may not clear the filter test as a whole, e.g. times ~1e9 s) and without it (the host proved
+    every tile clears). A batch holding a source at t0 = 1e9 s runs the first; its ordinary
+    sources must give the same bits as alone, where the second runs."""
+    torch = torch_gpu
+    from fastlisaresponse_amd.batch import BatchedResponse
+    from fastlisaresponse_amd.orbits import EqualArmlengthOrbits
+    orb = EqualArmlengthOrbits(t_max=1.0e9 + 2.0e6)
+    orb.configure(linear_interp_setup=True)
+    rng = np.random.default_rng(73)
+    lengths = [60000, 30000, 45000]
+    t0s = [0.0, 1.0e9, 2.5e7]
+    hs = [gb_waveform(n, 10.0, 1e-21, f, 1e-17, 0.7, 0.2, 0.4, t0=t0)
+          for n, f, t0 in zip(lengths, (3e-3, 4e-3, 6e-3), t0s)]
+    lams = rng.uniform(0, 6, 3)
+    betas = rng.uniform(-1, 1, 3)
+    br = BatchedResponse(0.1, tdi="2nd generation", orbits=orb, precision="mixed-c128")
+    mixed = br.plan(torch.as_tensor(np.concatenate(hs), device="cuda"), lengths, lams, betas, t0s)
+    mixed.run()
+    mixed.check_errors()
+    for s in (0, 2):
+        alone = br.plan(torch.as_tensor(hs[s], device="cuda"), [lengths[s]], [lams[s]], [betas[s]],
+                        [t0s[s]])
+        alone.run()
+        alone.check_errors()
+        assert torch.equal(torch.stack(mixed.channels(s)), torch.stack(alone.channels(0))), s
+    # and the far-time source against the oracle
+    oo = oracle_mod.OracleOrbits(*orb.pycppdetector_args)
+    _, exp = oracle_mod.response_pipeline(oo, hs[1], lams[1], betas[1], 10.0, 25, "2nd generation",
+                                          t0=t0s[1], x_base=orb.x_base, nthreads=8)
+    got = torch.stack(mixed.channels(1)).double().cpu().numpy()
+    for c in range(3):
+        assert rel_l2(got[c, 1000:-1000], exp[c, 1000:-1000]) <= TOL_C128, c
