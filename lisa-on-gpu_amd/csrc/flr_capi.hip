@@ -477,7 +477,8 @@ int make_fused_config(const flr_response *r, double dt, double fs, int flags, in
     pc.win_cap = tc.win_cap + pc.halo_lo + pc.halo_hi;
     const bool generic = stock.generation == 0 || r->order != 25;
     const bool lti = (flags & FLR_FLAG_MIXED) && !generic && stock.equal_arm;
-    if (fused_lds_bytes(pc, tc, generic, lti, passes) > kMaxLds)
+    const bool utv = (flags & FLR_FLAG_MIXED) && !generic && !stock.equal_arm;
+    if (fused_lds_bytes(pc, tc, generic, lti, passes, utv) > kMaxLds)
         return fail(FLR_ERR_UNSUPPORTED, "fused delay windows exceed LDS (sampling too fine)");
     return FLR_OK;
 }

@@ -237,8 +237,9 @@ __device__ __forceinline__ double stencil_exact(const double *x, double e, const
     return A * (B * x[1] + C * x[0] + D * sum);
 }
 
-template <int H>
-__device__ __forceinline__ double stencil_fast(const double *x, double e, const LagrangeView &lg)
+// x: a pointer at sample d, or any accessor with x[k] = sample d + k.
+template <int H, typename X>
+__device__ __forceinline__ double stencil_fast(const X &x, double e, const LagrangeView &lg)
 {
     const int h = H > 0 ? H : lg.h;
     constexpr int kUnroll = H > 0 ? H : 1;

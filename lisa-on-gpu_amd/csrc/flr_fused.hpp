@@ -187,6 +187,73 @@ __device__ __forceinline__ void lti_mfma(const FusedSource &src, int first, int 
     }
 }
 
+// The slowly-varying-arm TDI of one tile on the matrix cores (utv_unit): per group of 16
+// blocks of 16 samples (blocks aligned to the source's first TDI sample, as lti_mfma), the
+// three channels' F0 and F1 sums over their four base links -- 24 accumulations of 14
+// v_mfma_f64_16x16x4f64 each, the link series' operand shared by the pair's two filters --
+// then X_c = F0 part + (i - m) F1 part, A, E, T formed from X, Y, Z in FP64, and stored.
+// filt: [2][kUtvPairs][kFpad] (F0 then F1, padded as fpad); link rows canonical at ywin +
+// k * pitch.
+template <typename OT, int TPB>
+__device__ __forceinline__ void utv_mfma(const TdiConfig &tc, const FusedSource &src, int first,
+                                         int last, int m, int yw_lo, int yw_len, int kmin,
+                                         const double *ywin, int pitch, const double *filt)
+{
+    typedef double d4 __attribute__((ext_vector_type(4)));
+    const int T = src.tdi_start;
+    const int q0 = (first - T) >> 4, q1 = (last - T) >> 4;
+    const int ngc = (q1 - q0 + 16) >> 4;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nb = lane & 15, kq = lane >> 4;
+    const size_t n = (size_t)src.num_pts;
+    for (int g = wave; g < ngc; g += TPB / 64) {
+        const int qb = q0 + 16 * g + nb;
+        const int wb = T + 16 * min(qb, q1) + kmin - yw_lo;
+        d4 a0[3], a1[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a0[c] = a1[c] = d4{0.0, 0.0, 0.0, 0.0};
+        const double *fl = filt + kFpadLo + kq - nb;  // fl[p * kFpad + 4j] = F0_p[4j + kq - nb]
+#pragma unroll
+        for (int j = 0; j < 14; ++j) {
+            const int col = lti_col(min(max(wb + 4 * j + kq, 0), yw_len - 1));
+            static_for<6>([&](auto l) {  // canonical link l: one operand for its two pairs
+                const double yv = ywin[l * pitch + col];
+                static_for<3>([&](auto c) {
+                    static_for<kUtvLinks>([&](auto q) {
+                        if constexpr (stock::canon(stock::cyc(kUtvLink[q], c)) == l) {
+                            constexpr int p = c * kUtvLinks + q;
+                            a0[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(fl[p * kFpad + 4 * j], yv,
+                                                                         a0[c], 0, 0, 0);
+                            a1[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                                fl[(kUtvPairs + p) * kFpad + 4 * j], yv, a1[c], 0, 0, 0);
+                        }
+                    });
+                });
+                __asm__ volatile("" ::: "memory");  // one link's operands in flight at a time
+            });
+        }
+        if (qb > q1) continue;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = T + 16 * qb + 4 * v + kq;
+            if (i < first || i > last) continue;
+            const double di = (double)(i - m);
+            const double X = __builtin_fma(di, a1[0][v], a0[0][v]);
+            const double Y = __builtin_fma(di, a1[1][v], a0[1][v]);
+            const double Z = __builtin_fma(di, a1[2][v], a0[2][v]);
+            double o0 = X, o1 = Y, o2 = Z;
+            if (tc.aet) {
+                o0 = (Z - X) * kInvSqrt2;
+                o1 = (X - 2.0 * Y + Z) * kInvSqrt6;
+                o2 = (X + Y + Z) * kInvSqrt3;
+            }
+            store_out<OT>(src.out, i, o0 * src.scale);
+            store_out<OT>(src.out, n + i, o1 * src.scale);
+            store_out<OT>(src.out, 2 * n + i, o2 * src.scale);
+        }
+    }
+}
+
 // SAMPLE_PASS (composite-filter path): the per-sample test and fallback stencils for tiles that
 // cannot be cleared as a whole are compiled in. Without them (when the host has proved that
 // every tile clears, tc.lti_uniform) the kernel is smaller and its projection loop keeps its
@@ -396,6 +463,131 @@ void fused_lti_kernel(ProjConfig pc, TdiConfig tc, Batch<FusedSource> batch, Orb
     FLR_BATCH_LOOP(batch, (fused_tile<false, kModeLinear, NX, HT, float, PASSES, TPB, SAMPLE_PASS>(pc, tc, src, s_, tile, orb, lg, rm, smem, s_E)));
 }
 
+// The mixed form with unequal stock arms (order 25): fused_tile's staging and projection (link
+// rows canonical and padded), the tile's unit model (utv_unit) computed beside the projection,
+// then either the slowly-varying-arm filters on the matrix cores (utv_mfma) or, for a tile
+// that fails the model's tests, the per-sample stencils. LDS: the h window, whose space holds
+// the filters once the projection is done ([2][kUtvPairs][kFpad] doubles at most), then the
+// six padded rows.
+template <int NX, typename HT, int PASSES, int TPB>
+__device__ __forceinline__ void fused_tile_utv(const ProjConfig &pc, const TdiConfig &tc,
+                                               const FusedSource &src, int tile,
+                                               const OrbitView &orb, const LagrangeView &lg,
+                                               const RowMap &rm, double *smem, const double *s_E,
+                                               UtvShared &sh)
+{
+    constexpr int H = 13;
+    const int tile_len = tc.tile;
+    const int tid = threadIdx.x;
+    const int n = src.num_pts;
+    const int T = src.tdi_start;
+    const int i0 = (tc.zero_edges ? 0 : T) + tile * tile_len;
+    const int first = max(i0, T);
+    const int last = min(i0 + tile_len, n - T) - 1;
+    if (tc.zero_edges) {
+#pragma unroll
+        for (int r = 0; r < PASSES; ++r) {
+            const int i = i0 + r * TPB + tid;
+            if (i < n && (i < T || i >= n - T))
+                for (int c = 0; c < 3; ++c) store_out<float>(src.out, (size_t)c * n + i, 0.0);
+        }
+    }
+    if (first > last) return;
+    const int cap = tc.win_cap;
+    const int yw_lo = first - tc.halo_lo;
+    const int yw_len = last + tc.halo_hi - yw_lo + 1;
+    if (yw_len > cap || yw_lo < 0 || yw_lo + yw_len > n) {
+        for (int r = 0; r < PASSES; ++r) {
+            const int i = i0 + r * TPB + tid;
+            if (i < first || i > last) continue;
+            atomicOr(tc.err, kErrWindow);
+            for (int c = 0; c < 3; ++c) store_out<float>(src.out, (size_t)c * n + i, __builtin_nan(""));
+        }
+        return;
+    }
+    const int pitch = lti_pitch(cap);
+    const int hw_lo = max(0, yw_lo - pc.halo_lo);
+    const int hw_len = min(n, yw_lo + yw_len + pc.halo_hi) - hw_lo;
+    double2 *hwin = reinterpret_cast<double2 *>(smem);
+    double *filt = smem;  // after the projection
+    double *ywin = smem + max(2 * pc.win_cap, 2 * kUtvPairs * kFpad);  // [6][pitch]
+    const double t0_offset = src.t0 + (double)0 * pc.dt;
+    const int m = (first + last) >> 1;
+    // the times and light travel times at the tile's first, middle and last sample
+    if (tid < 3)
+        utv_reference_times(sh, tid, tid == 0 ? first : tid == 1 ? m : last, src.t0, tc.dt, orb, rm);
+    if (tid == 0) {
+        sh.kmin = 1 << 30;
+        sh.kmax = -(1 << 30);
+        sh.ok = 1;
+    }
+    for (int jj = tid; jj < hw_len; jj += TPB) hwin[jj] = load_h<HT>(src.h, hw_lo + jj);
+    __syncthreads();
+    if (tid < 3 * NX) utv_unit<NX>(sh, tid, first, m, last, t0_offset, tc.fs, lg);
+
+    bool bad = false;
+    for (int w = tid; w < yw_len; w += TPB) {
+        const int j = yw_lo + w;
+        int col = lti_col(w);
+        pin(col);
+        double *yc = ywin + col;
+        if (j < src.proj_start || j >= n - src.proj_start) {
+#pragma unroll
+            for (int r = 0; r < kNLinks; ++r) yc[r * pitch] = 0.0;
+        } else {
+            const double t = src.t0 + (double)j * pc.dt;
+            bad |= project_links<false, H, false, true, kELds>(
+                pc, src.k, src.u, src.v, t, t0_offset, orb, lg, hwin, hw_lo, hw_len, 0.0, s_E,
+                [&](int row, double val) { yc[row * pitch] = val; });
+        }
+    }
+    __syncthreads();  // rows and the unit model written; the h window is free
+
+    const int kmin = sh.kmin;
+    const bool tile_ok = sh.ok && sh.kmax - kmin + 1 <= kFir && kmin >= -tc.halo_lo &&
+                         sh.kmax <= tc.halo_hi;
+    if (tile_ok) {
+        const int nt = sh.kmax - kmin + 1;
+        for (int x = tid; x < kUtvPairs * kFpad; x += TPB) {
+            const int p = x / kFpad, k = x - p * kFpad - kFpadLo;
+            double f0 = 0.0, f1 = 0.0;
+            if (k >= 0 && k < nt) utv_filter_entry<NX>(sh, lg, s_E, p, k, f0, f1);
+            filt[x] = f0;
+            filt[kUtvPairs * kFpad + x] = f1;
+        }
+        __syncthreads();
+        utv_mfma<float, TPB>(tc, src, first, last, m, yw_lo, yw_len, kmin, ywin, pitch, filt);
+    } else {
+#pragma unroll 1
+        for (int r = 0; r < PASSES; ++r) {
+            const int i = i0 + r * TPB + tid;
+            if (i < first || i > last) continue;
+            const double t = src.t0 + (double)i * tc.dt;
+            const OrbitNode nd = orbit_node<false>(orb, t);
+            double L[6];
+#pragma unroll
+            for (int q = 0; q < kNLinks; ++q) L[q] = orbit_val(nd, rm.row[q]);
+            double acc[3] = {0.0, 0.0, 0.0};
+            bad |= !tdi_stock_unequal_padded<NX>(t, L, t0_offset, tc.fs, ywin, pitch, yw_lo, yw_len,
+                                                 lg, acc);
+            fused_store<float>(tc, src, i, acc, tc.aet);
+        }
+    }
+    if (bad) atomicOr(tc.err, kErrWindow);
+}
+
+template <int NX, typename HT, int PASSES, int TPB>
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4)))
+void fused_utv_kernel(ProjConfig pc, TdiConfig tc, Batch<FusedSource> batch, OrbitView orb,
+                      LagrangeView lg, RowMap rm)
+{
+    extern __shared__ double smem[];
+    __shared__ double s_E[kMaxHalf];
+    __shared__ UtvShared sh;
+    load_E_lds(lg, s_E);
+    FLR_BATCH_LOOP(batch, (fused_tile_utv<NX, HT, PASSES, TPB>(pc, tc, src, tile, orb, lg, rm, smem, s_E, sh)));
+}
+
 // Launch one fused variant over a batch: the TDI mode follows the table (stock + equal arms:
 // groups for exact stencils, linear merging for fast ones; otherwise unit by unit).
 template <bool EXACT, typename HT, typename OT, int PASSES>
@@ -427,8 +619,19 @@ hipError_t launch_fused_variant(const ProjConfig &pc, const TdiConfig &tc, const
             return st.generation == 1 ? FLR_FUSED(kEq, 8) : FLR_FUSED(kEq, 16);
         }
     }
-    if (use_stock(st, lg))
+    if (use_stock(st, lg)) {
+        if constexpr (!EXACT && std::is_same<OT, float>::value) {
+            // mixed, unequal arms: the slowly-varying-arm filters (fused_tile_utv)
+            constexpr int TPB = PASSES == 4 ? 2 * kBlock : kBlock;
+            constexpr int P = PASSES == 4 ? 2 : PASSES;
+            const size_t shu = fused_lds_bytes(pc, tc, false, false, PASSES, true);
+#define FLR_UTV(NXV) \
+    launch_tpb(fused_utv_kernel<NXV, HT, P, TPB>, nb, TPB, shu, stream, pc, tc, b, orb, lg, rm)
+            return st.generation == 1 ? FLR_UTV(8) : FLR_UTV(16);
+#undef FLR_UTV
+        }
         return st.generation == 1 ? FLR_FUSED(kModeStock, 8) : FLR_FUSED(kModeStock, 16);
+    }
     return FLR_FUSED(kModeGeneric, 0);
 #undef FLR_FUSED
 }

@@ -205,9 +205,10 @@ hipError_t launch_fused_c128(const ProjConfig &pc, const TdiConfig &tc,
                              const LagrangeView &lg, int tile_passes, int grid_per_cu,
                              const StockTdi &stock, hipStream_t stream);
 // Dynamic LDS of a fused launch: h window, six window rows (padded, plus the matrix-core
-// filters and per-sample flags, for the mixed equal-arm form), generic TDI scratch.
+// filters and per-sample flags, for the mixed equal-arm form), generic TDI scratch; utv: the
+// mixed unequal-arm form (h window or filters, padded rows).
 size_t fused_lds_bytes(const ProjConfig &pc, const TdiConfig &tc, bool generic, bool lti,
-                       int passes);
+                       int passes, bool utv = false);
 // Galactic-binary generator (flr_waveform.hip).
 hipError_t launch_gb(const flr_gb_source *sources, int num_sources, int64_t total, double dt,
                      int format, void *out, hipStream_t stream);

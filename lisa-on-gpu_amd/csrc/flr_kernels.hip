@@ -391,8 +391,12 @@ hipError_t launch_clear(int *word, hipStream_t stream)
 }
 
 size_t fused_lds_bytes(const ProjConfig &pc, const TdiConfig &tc, bool generic, bool lti,
-                       int passes)
+                       int passes, bool utv)
 {
+    if (utv)  // h window or the filters, then the padded rows (fused_tile_utv)
+        return ((size_t)std::max(2 * pc.win_cap, 2 * kUtvPairs * kFpad) +
+                (size_t)kNLinks * lti_pitch(tc.win_cap)) *
+               sizeof(double);
     const size_t doubles =
         lti ? (size_t)kNLinks * lti_pitch(tc.win_cap) + 2 * kFpad
             : (size_t)kNLinks * tc.win_cap + (generic ? (size_t)kNLinks * kBlock : 0);

@@ -524,4 +524,255 @@ __device__ __forceinline__ bool lti_matches(double t, int i, double L, double t0
     return ok;
 }
 
+// ------------------------------------------------------------------ slowly varying arms
+// With unequal, time-varying arms every unit u of the stock TDI has a position of its own,
+// delta_u(i) = d_u - i + 1 - e_u. Within a tile of some hundred samples the light travel times
+// are linear in t (the orbit tables are interpolated linearly, and where a tile crosses a node
+// the slope changes by ~1e-13), so delta_u(i) = delta_u(m) + sigma_u (i - m) about the tile's
+// middle sample m, with |sigma_u| ~ 1e-10 for LISA-like arm rates. Expanding each unit's stencil
+// weights w_r(e) to first order in e_u(i) - e_u(m) = -sigma_u (i - m), channel c of every output
+// sample of the tile is, over the channel's four base links q,
+//   X_c(i) = sum_q sum_k (F0_cq[k] + (i - m) F1_cq[k]) y_q[i + kmin + k],
+//   F0_cq[k] = sum_{u in (c,q)} sign_u w_{k+kmin-off_u}(e_u),
+//   F1_cq[k] = -sum_{u in (c,q)} sign_u sigma_u w'_{k+kmin-off_u}(e_u),
+// with off_u = d_u - m and e_u taken at m: 24 composite filters per tile, applied on the matrix
+// cores. The reference positions are computed in double-double (utv_unit), so unlike the
+// per-sample delays they carry no FP64 rounding of the times. The weights are the fast stencil's
+// (lagrange_weight_d), so the result differs from stencils at exact delays by the model's
+// position error (at most kLtiTol, tested per unit on the tile's two ends and middle), the
+// second-order term (|sigma| (b - a) / 2 <= kUtvDrift keeps it
+// below 1e-9 of the interpolated value) and a change of the A table's slope inside the tile's
+// range of e (tolerated up to kUtvKnot past a node of the table). A tile that fails any test
+// takes the per-sample stencils instead.
+constexpr double kUtvDrift = 1e-4;
+constexpr double kUtvKnot = 3e-8;
+constexpr int kUtvLinks = 4;                  // base links per channel (q)
+constexpr int kUtvPairs = 3 * kUtvLinks;      // (channel, base link) filters per kind
+constexpr int kUtvLink[kUtvLinks] = {13, 31, 12, 21};  // X's base links in order of first use
+
+// Base-link index q of X term x.
+__host__ __device__ constexpr int utv_q(int x)
+{
+    return stock::kX[x].link == 13 ? 0 : stock::kX[x].link == 31 ? 1 : stock::kX[x].link == 12 ? 2 : 3;
+}
+
+// Double-double arithmetic (value hi + lo, |lo| <= ulp(hi)/2): the model's reference positions
+// are computed to ~1e-20 relative, so the tile's model carries none of the FP64 rounding of a
+// time ~1e7 s (~1e-9 s) that every per-sample delay carries.
+struct DD {
+    double hi, lo;
+};
+__device__ __forceinline__ DD dd_two_sum(double a, double b)
+{
+    const double s = a + b;
+    const double bb = s - a;
+    return {s, (a - (s - bb)) + (b - bb)};
+}
+__device__ __forceinline__ DD dd_norm(double hi, double lo)
+{
+    const double s = hi + lo;
+    return {s, lo - (s - hi)};
+}
+__device__ __forceinline__ DD dd_add(DD a, DD b)
+{
+    const DD s = dd_two_sum(a.hi, b.hi);
+    return dd_norm(s.hi, s.lo + (a.lo + b.lo));
+}
+__device__ __forceinline__ DD dd_neg(DD a) { return {-a.hi, -a.lo}; }
+__device__ __forceinline__ DD dd_mul(DD a, double b)
+{
+    const double p = a.hi * b;
+    return dd_norm(p, __builtin_fma(a.hi, b, -p) + a.lo * b);
+}
+__device__ __forceinline__ DD dd_prod(double a, double b)
+{
+    const double p = a * b;
+    return {p, __builtin_fma(a, b, -p)};
+}
+
+// Per-tile model of the units, in LDS: the tile's first, middle and last sample time and the six
+// light travel times there (canonical order), in double-double; per unit (channel * NX + term)
+// off_u, e_u, sigma_u; the filters' tap range over all units and whether every unit passed.
+struct UtvShared {
+    double th[3], tl[3];
+    double Lh[3][6], Ll[3][6];
+    double off[48], e[48], sigma[48];
+    int kmin, kmax, ok;
+};
+
+// Reference sample s (0: first, 1: middle, 2: last) of the tile at sample i: t0 + i dt and the
+// orbit tables' linear interpolant L_k(t) = v_k + slope_k (t - dt w) of every link, in
+// double-double (window w from t as orbit_node picks it; the interpolant is continuous at nodes).
+__device__ __forceinline__ void utv_reference_times(UtvShared &sh, int s, int i, double t0,
+                                                    double dt, const OrbitView &orb,
+                                                    const RowMap &rm)
+{
+    const DD t = dd_add({t0, 0.0}, dd_prod((double)i, dt));
+    const double q = t.hi * orb.inv_dt;
+    const int w = !(q >= 0.0) ? 0 : (q >= (double)(orb.N - 1) ? orb.N - 2 : (int)q);
+    const DD tau = dd_add(t, dd_neg(dd_prod(orb.dt, (double)w)));
+    const double *p = orb.table + (size_t)w * kOrbitStride;
+    sh.th[s] = t.hi;
+    sh.tl[s] = t.lo;
+#pragma unroll
+    for (int k = 0; k < kNLinks; ++k) {
+        const DD L = dd_add(dd_mul(tau, p[kOrbitVals + rm.row[k]]), {p[rm.row[k]], 0.0});
+        sh.Lh[s][k] = L.hi;
+        sh.Ll[s][k] = L.lo;
+    }
+}
+
+// Unit u = c * NX + x of the tile [a, b] (middle m): its positions at a, m and b from the
+// double-double times (the chain subtracted in the reference's order, then the offset and the
+// scale by fs, LISAResponse.cu:314-341), and its model tests. Writes the unit's record and folds
+// its tap range and verdict into sh.
+template <int NX>
+__device__ __forceinline__ void utv_unit(UtvShared &sh, int u, int a, int m, int b,
+                                         double t0_offset, double fs, const LagrangeView &lg)
+{
+    constexpr int H = 13;
+    const int c = u / NX, x = u - c * NX;
+    const stock::Term &tm = stock::kX[x];
+    double pos[3];
+    int off_m = 0;
+    double e_m = 0.0;
+    for (int s = 0; s < 3; ++s) {
+        const int i = s == 0 ? a : s == 1 ? m : b;
+        DD delay = {sh.th[s], sh.tl[s]};
+        for (int k = 0; k < tm.len; ++k) {
+            const int l = stock::canon(stock::cyc(tm.chain[k], c));
+            delay = dd_add(delay, {-sh.Lh[s][l], -sh.Ll[s][l]});
+        }
+        const DD cf = dd_mul(dd_add(delay, {-t0_offset, 0.0}), fs);
+        // d = ceil(cf) - 1 and e = 1 + d - cf, the split of split_delay on the exact value
+        double cu = ceil(cf.hi);
+        if (cf.hi == cu && cf.lo > 0.0) cu += 1.0;
+        const int d = (int)cu - 1;
+        const double e = ((1.0 + (double)d) - cf.hi) - cf.lo;
+        pos[s] = (double)(d - i) + 1.0 - e;
+        if (s == 1) {
+            off_m = d - m;
+            e_m = e;
+        }
+    }
+    const double sigma = b > a ? (pos[2] - pos[0]) / (double)(b - a) : 0.0;
+    // (1) the line through the middle position with the chord's slope is off the positions at
+    // a and b by dev, and off any position in between by at most 3 dev (the positions are
+    // piecewise linear in i between the orbit nodes)
+    const double dev = fabs(pos[0] + sigma * (double)(m - a) - pos[1]);
+    bool ok = 3.0 * dev <= kLtiTol;
+    // (2) drift across the tile
+    const double drift = fabs(sigma) * (double)max(m - a, b - m);
+    ok = ok && drift <= kUtvDrift;
+    // (3) e over the tile, [e_m - drift, e_m + drift], within the A-table segment of e_m
+    const double s_m = e_m * lg.inv_deps;
+    const int ind = min(max((int)s_m, 0), lg.num_A - 2);
+    const double over = fmax(fmax((double)ind - (s_m - drift * lg.inv_deps),
+                                  (s_m + drift * lg.inv_deps) - (double)(ind + 1)),
+                             0.0) / lg.inv_deps;
+    ok = ok && over <= kUtvKnot;
+    sh.off[u] = (double)off_m;
+    sh.e[u] = e_m;
+    sh.sigma[u] = sigma;
+    atomicMin(&sh.kmin, off_m - (H - 1));
+    atomicMax(&sh.kmax, off_m + H);
+    if (!ok) sh.ok = 0;
+}
+
+// Weight of tap d + r (r = -(H-1) .. H) of the fast stencil at fraction e, and its derivative
+// in e: w_1 = A (1 - e), w_0 = A e, w_{j+1} = A D E_{j-1} / (j + e), w_{-j} = A D E_{j-1} /
+// (j + 1 - e), D = e (1 - e), A the linearly interpolated table (slope within e's segment).
+__device__ __forceinline__ void lagrange_weight_d(const LagrangeView &lg, const double *E_lds,
+                                                  double e, int r, double &w, double &dw)
+{
+    const double s = e * lg.inv_deps;
+    int ind = (int)s;
+    ind = ind < 0 ? 0 : (ind > lg.num_A - 2 ? lg.num_A - 2 : ind);
+    const double2 a = lg.Aslope[ind];
+    const double A = __builtin_fma(a.y, s - (double)ind, a.x);
+    const double dA = a.y * lg.inv_deps;
+    const double B = 1.0 - e;
+    const double D = e * B;
+    const double dD = 1.0 - 2.0 * e;
+    if (r == 1) {
+        w = A * B;
+        dw = __builtin_fma(dA, B, -A);
+    } else if (r == 0) {
+        w = A * e;
+        dw = __builtin_fma(dA, e, A);
+    } else {
+        const bool up = r > 1;
+        const int j = up ? r - 1 : -r;
+        const double inv_q = 1.0 / (up ? (double)j + e : (double)(j + 1) - e);
+        const double Ej = E_lds[j - 1];
+        const double AD = A * D;
+        w = AD * Ej * inv_q;
+        // d/de [A D / q] = (A' D + A D') / q -+ A D / q^2, q = j + e (up) or j + 1 - e
+        const double lin = __builtin_fma(dA, D, A * dD);
+        dw = Ej * inv_q * (up ? __builtin_fma(-AD, inv_q, lin) : __builtin_fma(AD, inv_q, lin));
+    }
+}
+
+// Entry k (0 <= k < kFir) of both filters of pair p = c * kUtvLinks + q of the tile's model.
+template <int NX>
+__device__ __forceinline__ void utv_filter_entry(const UtvShared &sh, const LagrangeView &lg,
+                                                 const double *E_lds, int p, int k, double &f0,
+                                                 double &f1)
+{
+    constexpr int H = 13;
+    const int c = p / kUtvLinks, q = p - c * kUtvLinks;
+    f0 = 0.0;
+    f1 = 0.0;
+    for (int x = 0; x < NX; ++x) {  // the pair's units in index order
+        if (utv_q(x) != q) continue;
+        const int u = c * NX + x;
+        const int r = k + sh.kmin - (int)sh.off[u];
+        if (r < -(H - 1) || r > H) continue;
+        double w, dw;
+        lagrange_weight_d(lg, E_lds, sh.e[u], r, w, dw);
+        const double sg = (double)stock::kX[x].sign;
+        f0 = __builtin_fma(sg, w, f0);
+        f1 = __builtin_fma(-sg * sh.sigma[u], dw, f1);
+    }
+}
+
+// Padded window column c + k of a row (c: the column of the stencil's sample d).
+struct PaddedAt {
+    const double *row;
+    int c;
+    __device__ __forceinline__ double operator[](int k) const { return row[lti_col(c + k)]; }
+};
+
+// tdi_stock_unequal (fast stencils) on padded rows in canonical link order (row k at ywin +
+// k * pitch): the per-sample path of the slowly-varying-arm form.
+template <int NX>
+__device__ __forceinline__ bool tdi_stock_unequal_padded(double t, const double (&L)[6],
+                                                         double t0_offset, double fs,
+                                                         const double *ywin, int pitch,
+                                                         int win_lo, int win_len,
+                                                         const LagrangeView &lg, double (&acc)[3])
+{
+    constexpr int H = 13;
+    bool ok = true;
+    static_for<3>([&](auto c) {
+        static_for<NX>([&](auto x) {
+            double delay = t;
+            pin(delay);
+            static_for<stock::kX[x].len>([&](auto s) { delay -= L[stock::chain_of(c, x, s)]; });
+            int d;
+            double e;
+            split_delay(delay, t0_offset, fs, d, e);
+            const bool bad = (d - win_lo - (H - 1) < 0) || (d - win_lo + H >= win_len);
+            ok = ok && !bad;
+            const PaddedAt at{ywin + stock::base_of(c, x) * pitch, bad ? H - 1 : d - win_lo};
+            double val = stencil_fast<H>(at, e, lg);
+            val = bad ? __builtin_nan("") : val;
+            val *= (double)stock::kX[x].sign;
+            acc[c] += val;
+            pin(acc[c]);
+        });
+    });
+    return ok;
+}
+
 }  // namespace flr

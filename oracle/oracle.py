@@ -392,5 +392,63 @@ def response_pipeline(orbits: OracleOrbits, input_in, lam, beta, dt, order, tdi,
     return y, np.stack([a, e, t])
 
 
+def tdi_extended_delays(orbits: OracleOrbits, y: np.ndarray, t0: float, dt: float, tdi,
+                        order: int = 25, tdi_start: int = 0, chan: str = "XYZ"):
+    """The TDI loop of get_tdi_delays (LISAResponse.cu:300-393) over the FP64 projections y
+    [6][N], with every sample's delay arithmetic -- t = t0 + i dt, the orbit tables' linear
+    interpolation at t, the chain subtraction, the offset and the scale by fs -- carried in
+    numpy.longdouble (x87 extended, 64-bit mantissa) before the integer/fraction split, and the
+    stencils in FP64 in interp_single's operation order. The per-sample delays of the FP64 path
+    round times of ~1e7 s to ~1e-9 s, which TDI amplifies in T wherever the arms differ; this is
+    the same response without that rounding, against which forms that compute their delays
+    differently are judged. Returns X, Y, Z (or A, E, T) on [tdi_start, N - tdi_start), zero
+    outside, like get_tdi_delays."""
+    LD = np.longdouble
+    y = np.asarray(y, dtype=np.float64)
+    n = y.shape[1]
+    fs = 1.0 / dt
+    A_in, deps, num_A, E_in = lagrange_tables(order)
+    h = int((order + 1) / 2)
+    o = orbits
+    odt, oN = LD(o.s.dt), int(o.s.N)
+    ltt_tab = o.ltt.reshape(oN, 6)
+    row = {int(o.s.links[r]): r for r in range(6)}
+    ii = np.arange(tdi_start, n - tdi_start)
+    t = LD(t0) + ii.astype(LD) * LD(dt)
+    q = (t / odt).astype(np.float64)
+    w = np.clip(np.where(q >= 0.0, q, 0.0).astype(np.int64), 0, oN - 2)
+    tau = t - odt * w.astype(LD)
+    L = {}
+    for link, r in row.items():
+        down = ltt_tab[w, r].astype(LD)
+        up = ltt_tab[w + 1, r].astype(LD)
+        L[link] = (up - down) / odt * tau + down
+    combos = stock_combinations(tdi) if isinstance(tdi, str) else tdi
+    tables = tdi_tables(combos)
+    out = np.zeros((3, n))
+    for u in range(tables["num_units"]):
+        c = int(tables["channels"][u])
+        delay = t.copy()
+        for k in range(tables["unit_starts"][u], tables["unit_starts"][u] + tables["unit_lengths"][u]):
+            link = int(tables["tdi_link_combinations"][k])
+            if link != -11:
+                delay = delay - L[link]
+        cf = (delay - LD(t0)) * LD(fs)
+        d = np.ceil(cf).astype(np.int64) - 1
+        e = (LD(1) + d.astype(LD) - cf).astype(np.float64)
+        x = y[row[int(tables["tdi_base_link"][u])]]
+        ind = np.minimum((e / deps).astype(np.int64), num_A - 2)  # e = 1 exactly: last segment
+        frac = (e - ind * deps) / deps
+        A = A_in[ind] * (1.0 - frac) + A_in[ind + 1] * frac
+        acc = np.zeros(ii.size)
+        for j in range(1, h):
+            acc += E_in[j - 1] * (x[d + 1 + j] / (j + e) + x[d - j] / (j + (1 - e)))
+        val = A * ((1.0 - e) * x[d + 1] + e * x[d] + (e * (1.0 - e)) * acc)
+        out[c, tdi_start:n - tdi_start] += tables["tdi_signs"][u] * val
+    if chan == "XYZ":
+        return out
+    return np.stack(aet(*out))
+
+
 def max_threads() -> int:
     return lib().oracle_max_threads()

@@ -15,7 +15,7 @@
 import numpy as np
 import pytest
 
-from conftest import YRSID_SI, gb_waveform, rel_l2
+from conftest import YRSID_SI, check_mixed, gb_waveform, rel_l2, unequal_arms
 
 pytestmark = pytest.mark.gpu
 
@@ -145,13 +145,12 @@ def test_mixed_kernel_vs_oracle_on_rounded_input(torch_gpu, orbits, oracle_orbit
         lam, beta, t0 = params[s]
         hq = h32_host[o:o + n].astype(np.complex128) * scales[s]
         o += n
-        _, exp = oracle_mod.response_pipeline(oracle_orbits, hq, lam, beta, 10.0, 25,
+        y, exp = oracle_mod.response_pipeline(oracle_orbits, hq, lam, beta, 10.0, 25,
                                               "2nd generation", t0=t0, x_base=orbits.x_base,
                                               nthreads=8)
         got = torch.stack(plan.channels(s)).cpu().numpy().astype(np.float64)
-        for c in range(3):
-            r = rel_l2(got[c, 1000:-1000], exp[c, 1000:-1000])
-            assert r <= TOL_MIXED_KERNEL, (s, c, r)
+        check_mixed(oracle_mod, oracle_orbits, got, y, exp, t0, 10.0, "2nd generation", "AET",
+                    TOL_MIXED_KERNEL, 1000, (s,), unequal_arms(orbits))
 
 
 @pytest.mark.parametrize("tdi,chan,dt", [("1st generation", "XYZ", 10.0),
@@ -175,13 +174,11 @@ def test_mixed_generations_channels_and_sampling(torch_gpu, orbits, oracle_orbit
     plan.run()
     plan.check_errors()
     hq = h32.cpu().numpy().astype(np.complex128) * scales[0]
-    _, exp = oracle_mod.response_pipeline(oracle_orbits, hq, 0.9, -0.4, dt, 25, tdi, t0=1.0e6,
+    y, exp = oracle_mod.response_pipeline(oracle_orbits, hq, 0.9, -0.4, dt, 25, tdi, t0=1.0e6,
                                           tdi_chan=chan, x_base=orbits.x_base, nthreads=8)
     got = torch.stack(plan.channels(0)).cpu().numpy().astype(np.float64)
-    T = br.tdi_start_ind
-    for c in range(3):
-        r = rel_l2(got[c, T:-T], exp[c, T:-T])
-        assert r <= TOL_MIXED_KERNEL, (c, r)
+    check_mixed(oracle_mod, oracle_orbits, got, y, exp, 1.0e6, dt, tdi, chan, TOL_MIXED_KERNEL,
+                br.tdi_start_ind, (), unequal_arms(orbits))
 
 
 def test_mixed_vs_fp64_oracle_at_5mhz(torch_gpu, orbits, oracle_orbits, oracle_mod):
@@ -235,13 +232,12 @@ def test_q31_kernel_vs_oracle_on_dequantised_input(torch_gpu, orbits, oracle_orb
         lam, beta, t0 = params[s]
         hq = (q[o:o + n, 0] + 1j * q[o:o + n, 1]) * scales[s]
         o += n
-        _, exp = oracle_mod.response_pipeline(oracle_orbits, hq, lam, beta, 10.0, 25,
+        y, exp = oracle_mod.response_pipeline(oracle_orbits, hq, lam, beta, 10.0, 25,
                                               "2nd generation", t0=t0, x_base=orbits.x_base,
                                               nthreads=8)
         got = torch.stack(plan.channels(s)).cpu().numpy().astype(np.float64)
-        for c in range(3):
-            r = rel_l2(got[c, 1000:-1000], exp[c, 1000:-1000])
-            assert r <= TOL_MIXED_KERNEL, (s, c, r)
+        check_mixed(oracle_mod, oracle_orbits, got, y, exp, t0, 10.0, "2nd generation", "AET",
+                    TOL_MIXED_KERNEL, 1000, (s,), unequal_arms(orbits))
 
 
 def test_q31_vs_fp64_oracle_at_5mhz(torch_gpu, orbits, oracle_orbits, oracle_mod):
@@ -279,11 +275,10 @@ def test_c128_vs_fp64_oracle_at_5mhz(torch_gpu, orbits, oracle_orbits, oracle_mo
     A, E, T = r.get_tdi_delays()
     assert A.dtype == torch.float32
     got = torch.stack([A, E, T]).cpu().numpy().astype(np.float64)
-    _, exp = oracle_mod.response_pipeline(oracle_orbits, h, 1.2, 0.3, 10.0, 25, "2nd generation",
+    y, exp = oracle_mod.response_pipeline(oracle_orbits, h, 1.2, 0.3, 10.0, 25, "2nd generation",
                                           x_base=orbits.x_base, nthreads=8)
-    for c in range(3):
-        r_c = rel_l2(got[c, 1000:-1000], exp[c, 1000:-1000])
-        assert r_c <= TOL_C128, (c, r_c)
+    check_mixed(oracle_mod, oracle_orbits, got, y, exp, 0.0, 10.0, "2nd generation", "AET",
+                TOL_C128, 1000, (), unequal_arms(orbits))
 
 
 def test_bound_filters_follow_table_changes(torch_gpu, orbits):
@@ -400,13 +395,12 @@ def test_c128_batch_vs_oracle(torch_gpu, orbits, oracle_orbits, oracle_mod, tdi)
         assert plan.out_all.dtype == torch.float32
         for s, n in enumerate(lengths):
             lam, beta, t0 = params[s]
-            _, exp = oracle_mod.response_pipeline(oracle_orbits, hs[s], lam, beta, 10.0, 25, tdi,
+            y, exp = oracle_mod.response_pipeline(oracle_orbits, hs[s], lam, beta, 10.0, 25, tdi,
                                                   t0=t0, tdi_chan=chan, x_base=orbits.x_base,
                                                   nthreads=8)
             got = torch.stack(plan.channels(s)).cpu().numpy().astype(np.float64)
-            for c in range(3):
-                r = rel_l2(got[c, 1000:-1000], exp[c, 1000:-1000])
-                assert r <= TOL_C128, (chan, s, c, r)
+            check_mixed(oracle_mod, oracle_orbits, got, y, exp, t0, 10.0, tdi, chan, TOL_C128,
+                        1000, (chan, s), unequal_arms(orbits))
 
 
 def test_fused_argument_checks(torch_gpu, orbits):
@@ -431,7 +425,9 @@ def test_fused_argument_checks(torch_gpu, orbits):
 @pytest.mark.parametrize("kind", ["mixed", "mixed-q31", "mixed-c128", "fused-fp64", "unfused"])
 def test_launch_shapes_give_identical_results(torch_gpu, orbits, kind):
     """Tile length and grid density change speed only: every shape the autotuner may pick gives
-    the same bits."""
+    the same bits -- except the mixed forms with unequal arms, whose composite filters are
+    built per tile about the tile's middle sample: there the shapes agree within the float32
+    outputs' rounding on A and E (the models differ by second-order terms, ~1e-12 of y)."""
     torch = torch_gpu
     from fastlisaresponse_amd.autotune import apply, candidates
     from fastlisaresponse_amd.batch import BatchedResponse, pack_mixed
@@ -459,6 +455,13 @@ def test_launch_shapes_give_identical_results(torch_gpu, orbits, kind):
         plan.check_errors()
         if ref is None:
             ref = plan.out_all.clone()
+        elif kind.startswith("mixed") and unequal_arms(orbits):
+            for s in range(len(lengths)):
+                o, n = int(plan.offsets[s]), int(plan.lengths[s])
+                a = torch.stack(plan.channels(s)).double()
+                b = ref[3 * o:3 * (o + n)].view(3, n).double()
+                rel = ((a - b).norm(dim=1) / b.norm(dim=1)).cpu().numpy()
+                assert np.all(rel[:2] <= 1e-8) and rel[2] <= 1e-6, (shape, s, rel)
         else:
             assert torch.equal(plan.out_all, ref), shape
     res = plan.tune(use_cache=False, reps=1)

@@ -348,3 +348,60 @@ def test_planted_mistakes_are_detected(oracle_mod, mutant_dir, monkeypatch, muta
         not np.isclose(A_[0], -1 / np.sqrt(2.0)),
     ]
     assert any(failures), f"mutant {mutant} went undetected"
+
+
+# ------------------------------------------------------------------ extended-precision delays
+
+def test_extended_delay_tdi_matches_delayed_sums(oracle_mod):
+    """tdi_extended_delays against the analytic delayed sums (frozen unequal arms), the closed
+    form the FP64 TDI loop is pinned by, to the same bound."""
+    from fastlisaresponse_amd.tables import stock_tdi
+    L = np.array([8.31, 8.33, 8.35, 8.32, 8.34, 8.36])
+    o, _ = frozen_orbits(L)
+    oo = oracle_mod.OracleOrbits(*o.pycppdetector_args)
+    n, dt, T = 5000, 10.0, 1000
+    t = np.arange(n) * dt
+    rng = np.random.default_rng(7)
+    f = rng.uniform(1e-3, 8e-3, 6)
+    ph = rng.uniform(0, 6, 6)
+    amp = rng.uniform(0.5, 2, 6)
+    ylink = lambda r, s: amp[r] * np.sin(2 * np.pi * f[r] * s + ph[r])
+    y = np.stack([ylink(r, t) for r in range(6)])
+    combos = stock_tdi("2nd generation")
+    xyz = oracle_mod.tdi_extended_delays(oo, y, 0.0, dt, combos, tdi_start=T)
+    tabs = oracle_mod.tdi_tables(combos)
+    row = {l: r for r, l in enumerate(o.LINKS)}
+    ts = t[T:n - T]
+    for c in range(3):
+        exp = np.zeros_like(ts)
+        for u_ in np.where(tabs["channels"] == c)[0]:
+            s0, ln = tabs["unit_starts"][u_], tabs["unit_lengths"][u_]
+            delay = sum(L[row[l]] for l in tabs["tdi_link_combinations"][s0:s0 + ln] if l != -11)
+            exp += tabs["tdi_signs"][u_] * ylink(row[int(tabs["tdi_base_link"][u_])], ts - delay)
+        assert rel_l2(xyz[c, T:n - T], exp) < 2e-6
+    assert np.all(xyz[:, :T] == 0.0) and np.all(xyz[:, n - T:] == 0.0)
+
+
+def test_extended_delay_tdi_differs_from_fp64_by_time_rounding_only(oracle_mod):
+    """On the unequal-arm cartwheel the FP64 oracle and the extended-delay TDI of the same y
+    differ by the FP64 rounding of each sample's delay, which grows with |t|: XYZ within 1e-11
+    at t ~ 1e5 s; at t ~ 2.5e7 s (ulp 3.7e-9 s) about 1e-10 on X, Y, Z and ~1e-7 on the null
+    channel T, where it does not cancel."""
+    from fastlisaresponse_amd.orbits import CartwheelOrbits
+    from conftest import gb_waveform
+    o = CartwheelOrbits(t_max=3e7)
+    o.configure(linear_interp_setup=True)
+    oo = oracle_mod.OracleOrbits(*o.pycppdetector_args)
+    n, T = 30000, 1000
+    res = {}
+    for t0 in (0.0, 2.5e7):
+        h = gb_waveform(n, 10.0, 1e-21, 5e-3, 1e-17, 0.7, 0.2, 0.4, t0=t0)
+        y, xyz = oracle_mod.response_pipeline(oo, h, 1.0, 0.3, 10.0, 25, "2nd generation", t0=t0,
+                                              x_base=o.x_base, tdi_chan="XYZ", nthreads=4)
+        ext = oracle_mod.tdi_extended_delays(oo, y, t0, 10.0, "2nd generation", tdi_start=T)
+        res[t0] = [rel_l2(xyz[c, T:n - T], ext[c, T:n - T]) for c in range(3)]
+        ta, te = oracle_mod.aet(*xyz), oracle_mod.aet(*ext)
+        res[t0].append(rel_l2(ta[2][T:n - T], te[2][T:n - T]))
+    assert max(res[0.0][:3]) < 1e-11, res
+    assert 1e-11 < max(res[2.5e7][:3]) < 1e-9, res
+    assert 1e-8 < res[2.5e7][3] < 1e-6, res
