@@ -211,6 +211,20 @@ int flr_fused_bind(flr_batch *batch, const flr_source *sources, int num_sources,
                    const void *h_all, void *out_all, void *stream);
 /* Enqueue the batch's launches on `stream` (projection then TDI, or the one fused launch). */
 int flr_batch_run(flr_batch *batch, void *stream);
+/* Fused inner products against a data series: no counterpart in the reference, where the
+ * response is written out and utils/utility.py:16-82 (get_overlap) or a likelihood reads it
+ * back. Runs the batch's fused launch without storing its outputs: every output h (FP64, before
+ * the float32 rounding) meets data_all at the same place -- data_all is laid out as the bound
+ * out_all and has its element type (float32 for the mixed forms) -- and per source s and output
+ * channel c, over [tdi_start_ind, num_pts - tdi_start_ind), sums[(s*3 + c)*4 + k] (device
+ * memory, double) receives k = 0: sum d h, 1: sum h^2, 2: sum h, 3: sum (-1)^(i - tdi_start_ind)
+ * h. From these the white-noise inner products and log-likelihood follow, and the real part of
+ * get_overlap's rfft overlap by Parseval (the DC and Nyquist bins from sums 2 and 3). Mixed
+ * fused batches (FLR_FLAG_MIXED) with a stock TDI table at order 25 (FLR_ERR_UNSUPPORTED
+ * otherwise); deterministic (fixed reduction order). The per-tile sums live in a device buffer
+ * of the batch, allocated at the first call and when the batch's tile count grows (so the
+ * first call must not be under stream capture); later calls allocate nothing. */
+int flr_fused_inner(flr_batch *batch, const void *data_all, double *sums, void *stream);
 /* Launch shape of a batch (no counterpart in the reference): tile_samples window samples per
  * workgroup tile (fused mixed: 256, 512 or 1024; fused FP64: 512; unfused: 256; 0 keeps the
  * current; a fused tile outputs tile_samples minus the TDI delay halos, so that its projection

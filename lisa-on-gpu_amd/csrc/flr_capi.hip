@@ -174,6 +174,9 @@ struct flr_batch {
     bool lti_uniform = false;        // every bound source clears the composite-filter tile test
     uint64_t uniform_version = ~0ull;  // response version and tile length lti_uniform holds for
     int uniform_tile = 0;
+    const void *out_base = nullptr;  // fused: the bound out_all (inner-product launches)
+    double *d_inner = nullptr;       // fused inner products: per tile [3][kInnerSums] sums
+    int64_t inner_cap = 0;           // tiles d_inner holds
 };
 
 namespace {
@@ -864,6 +867,7 @@ void flr_batch_destroy(flr_batch *b)
     if (b->d_fused) (void)hipFree(b->d_fused);
     if (b->d_fused_tiles) (void)hipFree(b->d_fused_tiles);
     if (b->d_lti) (void)hipFree(b->d_lti);
+    if (b->d_inner) (void)hipFree(b->d_inner);
     if (b->h_fused) (void)hipHostFree(b->h_fused);
     if (b->h_fused_tiles) (void)hipHostFree(b->h_fused_tiles);
     delete b;
@@ -1133,6 +1137,7 @@ int flr_fused_bind(flr_batch *b, const flr_source *sources, int num_sources, con
     b->num_sources = num_sources;
     b->samples = samples;
     b->fused_tiles = ft[num_sources];
+    b->out_base = out_all;
     if (!same_refs) b->lti_version = ~0ull;  // new sources: their filters are built now
     b->uniform_version = ~0ull;               // and the tile-test proof is redone at the run
     const int fst = lti_refresh(b, st);
@@ -1164,6 +1169,95 @@ int flr_fused_create(flr_response *r, const flr_source *sources, int num_sources
     return FLR_OK;
 }
 
+namespace {
+
+// The fused launch of a bound batch on stream s: outputs, or (data set) the inner products of
+// the outputs with data (laid out as out_all) into sums [num_sources][3][kInnerSums].
+int fused_launch(flr_batch *b, hipStream_t s, const void *data, double *sums)
+{
+    flr_response *r = b->r;
+    const bool exact = (b->flags & FLR_FLAG_EXACT) != 0;
+    ProjConfig pc;
+    TdiConfig tc;
+    StockTdi stock;
+    int st;
+    if (b->num_sources < 1) return fail(FLR_ERR_INVALID_ARGUMENT, "no sources bound to the batch");
+    if ((st = make_fused_config(r, b->dt, b->fs, b->flags, b->tile_passes, pc, tc, stock)))
+        return st;
+    bool cap = false;
+    if ((st = stream_capturing(s, cap))) return st;
+    if (data) {
+        if (!(b->flags & FLR_FLAG_MIXED) || stock.generation == 0 || r->order != 25)
+            return fail(FLR_ERR_UNSUPPORTED, "fused inner products run in the mixed-precision "
+                                             "kernels with a stock TDI table at order 25");
+    }
+    if ((st = fused_retile(b, tc.tile, s))) return st;
+    if (data && b->fused_tiles > b->inner_cap) {
+        if (cap)
+            return fail(FLR_ERR_UNSUPPORTED, "the inner-product buffer must grow: call "
+                                             "flr_fused_inner once outside stream capture first");
+        if (b->d_inner) {
+            FLR_HIP(hipStreamSynchronize(s));
+            if (b->ran_recorded) FLR_HIP(hipEventSynchronize(b->ran));
+            (void)hipFree(b->d_inner);
+            b->d_inner = nullptr;
+            b->inner_cap = 0;
+        }
+        FLR_HIP(hipMalloc(&b->d_inner, sizeof(double) * 3 * kInnerSums * (size_t)b->fused_tiles));
+        b->inner_cap = b->fused_tiles;
+    }
+    if ((st = lti_refresh(b, s))) return st;
+    Batch<FusedSource> bf{b->d_fused, b->d_fused_tiles, b->num_sources, b->fused_tiles};
+    tc.lti_fir = b->d_lti;
+    if ((b->flags & FLR_FLAG_MIXED) && stock.generation != 0 && stock.equal_arm &&
+        r->order == 25) {
+        if (b->uniform_version != r->version || b->uniform_tile != tc.tile) {
+            // the staging copy holds the bound sources (its upload may still be in flight:
+            // reading it on the host is fine)
+            bool all = true;
+            for (int q = 0; q < b->num_sources && all; ++q)
+                all = lti_source_uniform(r->orbits, b->h_fused[q], b->dt, b->fs,
+                                         stock.generation == 1 ? 8 : 16, tc.halo_lo, tc.halo_hi);
+            b->lti_uniform = all;
+            b->uniform_version = r->version;
+            b->uniform_tile = tc.tile;
+        }
+        tc.lti_uniform = b->lti_uniform ? 1 : 0;
+    }
+    if (data) {
+        tc.inner_part = b->d_inner;
+        tc.inner_out = static_cast<const char *>(b->out_base);
+        tc.inner_data = static_cast<const char *>(data);
+    }
+    hipError_t e = launch_fused_batch(pc, tc, bf, r->orbits->view(), r->lagrange(), exact,
+                                      (b->flags & FLR_FLAG_MIXED) != 0,
+                                      (b->flags & FLR_FLAG_Q31) ? kInQ31
+                                      : (b->flags & FLR_FLAG_C128) ? kInC128 : kInC64,
+                                      b->tile_passes, b->grid_per_cu, stock, s);
+    if (e != hipSuccess) return hip_fail(e, "fused batch launch");
+    if (data) {
+        e = launch_inner_reduce(b->d_inner, b->d_fused_tiles, b->num_sources, sums, s);
+        if (e != hipSuccess) return hip_fail(e, "inner-product reduction launch");
+    }
+    // an event recorded inside a capture cannot be waited on outside it: replays of a
+    // captured run are ordered against later binds by the caller
+    if (!cap) {
+        FLR_HIP(hipEventRecord(b->ran, s));
+        b->ran_recorded = true;
+    }
+    return FLR_OK;
+}
+
+}  // namespace
+
+int flr_fused_inner(flr_batch *b, const void *data_all, double *sums, void *stream)
+{
+    if (!b || !b->r || !b->fused) return fail(FLR_ERR_INVALID_ARGUMENT, "not a fused batch");
+    if (!data_all || !sums) return fail(FLR_ERR_INVALID_ARGUMENT, "NULL data or sums");
+    DeviceGuard g(b->r->device);
+    return fused_launch(b, (hipStream_t)stream, data_all, sums);
+}
+
 int flr_batch_run(flr_batch *b, void *stream)
 {
     if (!b || !b->r) return fail(FLR_ERR_INVALID_ARGUMENT, "batch is NULL");
@@ -1175,47 +1269,7 @@ int flr_batch_run(flr_batch *b, void *stream)
     TdiConfig tc;
     StockTdi stock;
     int st;
-    if (b->fused) {
-        if (b->num_sources < 1) return fail(FLR_ERR_INVALID_ARGUMENT, "no sources bound to the batch");
-        if ((st = make_fused_config(r, b->dt, b->fs, b->flags, b->tile_passes, pc, tc, stock)))
-            return st;
-        if ((st = fused_retile(b, tc.tile, s))) return st;
-        if ((st = lti_refresh(b, s))) return st;
-        Batch<FusedSource> bf{b->d_fused, b->d_fused_tiles, b->num_sources, b->fused_tiles};
-        tc.lti_fir = b->d_lti;
-        if ((b->flags & FLR_FLAG_MIXED) && stock.generation != 0 && stock.equal_arm &&
-            r->order == 25) {
-            if (b->uniform_version != r->version || b->uniform_tile != tc.tile) {
-                // the staging copy holds the bound sources (its upload may still be in flight:
-                // reading it on the host is fine)
-                bool all = true;
-                for (int q = 0; q < b->num_sources && all; ++q)
-                    all = lti_source_uniform(r->orbits, b->h_fused[q], b->dt, b->fs,
-                                             stock.generation == 1 ? 8 : 16, tc.halo_lo,
-                                             tc.halo_hi);
-                b->lti_uniform = all;
-                b->uniform_version = r->version;
-                b->uniform_tile = tc.tile;
-            }
-            tc.lti_uniform = b->lti_uniform ? 1 : 0;
-        }
-        hipError_t e = launch_fused_batch(pc, tc, bf, r->orbits->view(), r->lagrange(), exact,
-                                          (b->flags & FLR_FLAG_MIXED) != 0,
-                                          (b->flags & FLR_FLAG_Q31) ? kInQ31
-                                          : (b->flags & FLR_FLAG_C128) ? kInC128 : kInC64,
-                                          b->tile_passes,
-                                          b->grid_per_cu, stock, s);
-        if (e != hipSuccess) return hip_fail(e, "fused batch launch");
-        // an event recorded inside a capture cannot be waited on outside it: replays of a
-        // captured run are ordered against later binds by the caller
-        bool cap = false;
-        if ((st = stream_capturing(s, cap))) return st;
-        if (!cap) {
-            FLR_HIP(hipEventRecord(b->ran, s));
-            b->ran_recorded = true;
-        }
-        return FLR_OK;
-    }
+    if (b->fused) return fused_launch(b, s, nullptr, nullptr);
     st = make_proj_config(r, b->dt, b->fs, b->flags, pc);
     if (st) return st;
     st = make_tdi_config(r, b->dt, b->fs, b->flags, tc, stock);

@@ -46,9 +46,91 @@ __device__ __forceinline__ void store_out(void *out, size_t idx, double v)
     reinterpret_cast<OT *>(out)[idx] = (OT)v;
 }
 
+// ------------------------------------------------------------------ inner-product epilogue
+// In an inner-product launch (tc.inner_part set) a tile's outputs are not stored: each output
+// h (FP64, before the float32 rounding, times the source's scale) meets the data series d at
+// the same place, and per channel the tile sums d h, h h, h and (-1)^(i - T) h -- per thread
+// (InnerAcc), then per wave (a fixed butterfly) into per-wave LDS slots (InnerShared), then
+// per tile in wave order into inner_part [tile][3][kInnerSums]; launch_inner_reduce adds the
+// tiles of each source in a fixed order. The result does not depend on timing.
+struct InnerAcc {
+    double a[3][kInnerSums];
+};
+struct InnerShared {
+    double v[3][kInnerSums][16];  // [channel][sum][wave], up to 16 waves per workgroup
+};
+
+__device__ __forceinline__ void inner_zero(InnerAcc &ia)
+{
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int k = 0; k < kInnerSums; ++k) ia.a[c][k] = 0.0;
+}
+
+// Channel c's output v (before the source's scale) at sample i, into a.
+template <typename OT>
+__device__ __forceinline__ void inner_add(const TdiConfig &tc, const FusedSource &src, int c,
+                                          int i, double v, double (&a)[kInnerSums])
+{
+    const OT *d = reinterpret_cast<const OT *>(
+        tc.inner_data + (reinterpret_cast<const char *>(src.out) - tc.inner_out));
+    const double w = v * src.scale;
+    const double x = (double)d[(size_t)c * src.num_pts + i];
+    a[0] = __builtin_fma(x, w, a[0]);
+    a[1] = __builtin_fma(w, w, a[1]);
+    a[2] += w;
+    a[3] += ((i - src.tdi_start) & 1) ? -w : w;
+}
+
+// The wave's sums of a (every lane of the wave takes part) added to its slots of channel c.
+__device__ __forceinline__ void inner_commit(InnerShared &sh, int c, const double (&a)[kInnerSums])
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < kInnerSums; ++k) {
+        double v = a[k];
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0) sh.v[c][k][wave] += v;
+    }
+}
+
+__device__ __forceinline__ void inner_commit_all(InnerShared &sh, const InnerAcc &ia)
+{
+#pragma unroll
+    for (int c = 0; c < 3; ++c) inner_commit(sh, c, ia.a[c]);
+}
+
+__device__ __forceinline__ void inner_clear(InnerShared &sh)
+{
+    double *v = &sh.v[0][0][0];
+    for (int x = threadIdx.x; x < 3 * kInnerSums * 16; x += blockDim.x) v[x] = 0.0;
+}
+
+// End of tile g: its record from the wave slots (in wave order), which are cleared for the
+// next tile (ordered before its commits by the persistent loop's barrier).
+__device__ __forceinline__ void inner_flush(const TdiConfig &tc, InnerShared &sh, int g)
+{
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t < 3 * kInnerSums) {
+        const int c = t / kInnerSums, k = t - c * kInnerSums;
+        double sum = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+            sum += sh.v[c][k][w];
+            sh.v[c][k][w] = 0.0;
+        }
+        tc.inner_part[(size_t)g * (3 * kInnerSums) + t] = sum;
+    }
+}
+
+// Store the three outputs of sample i (or, with ia in an inner-product launch, add them to
+// its sums).
 template <typename OT>
 __device__ __forceinline__ void fused_store(const TdiConfig &tc, const FusedSource &src, int i,
-                                            const double (&acc)[3], bool xyz_to_aet)
+                                            const double (&acc)[3], bool xyz_to_aet,
+                                            InnerAcc *ia = nullptr)
 {
     const size_t n = (size_t)src.num_pts;
     double o[3];
@@ -60,6 +142,11 @@ __device__ __forceinline__ void fused_store(const TdiConfig &tc, const FusedSour
     } else {
 #pragma unroll
         for (int c = 0; c < 3; ++c) o[c] = acc[c];
+    }
+    if (ia) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) inner_add<OT>(tc, src, c, i, o[c], ia->a[c]);
+        return;
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) store_out<OT>(src.out, c * n + i, o[c] * src.scale);
@@ -149,11 +236,13 @@ __global__ __launch_bounds__(kBlock) void lti_filter_kernel(TdiConfig tc,
 // outside the tile, and samples left to the per-sample path (lti_ok = 0 unless all_ok), are
 // not stored.
 template <typename OT, int TPB>
-__device__ __forceinline__ void lti_mfma(const FusedSource &src, int first, int last, int i0,
-                                         int yw_lo, int yw_len, int kmin, const double *ywin,
-                                         int pitch, const double *fpad,
-                                         const unsigned char *lti_ok, bool all_ok)
+__device__ __forceinline__ void lti_mfma(const TdiConfig &tc, const FusedSource &src, int first,
+                                         int last, int i0, int yw_lo, int yw_len, int kmin,
+                                         const double *ywin, int pitch, const double *fpad,
+                                         const unsigned char *lti_ok, bool all_ok,
+                                         InnerShared *ish)
 {
+    const bool inner = ish != nullptr;  // (inner-product kernels only)
     typedef double d4 __attribute__((ext_vector_type(4)));
     const int T = src.tdi_start;
     const int q0 = (first - T) >> 4, q1 = (last - T) >> 4;  // first >= T
@@ -177,13 +266,20 @@ __device__ __forceinline__ void lti_mfma(const FusedSource &src, int first, int 
                 __asm__ volatile("" ::: "memory");  // one operand pair in flight: no register growth
             }
         }
-        if (qb > q1) continue;
+        double a[kInnerSums] = {0.0, 0.0, 0.0, 0.0};
+        if (qb <= q1) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int i = T + 16 * qb + 4 * v + kq;
-            if (i >= first && i <= last && (all_ok || lti_ok[i - i0]))
-                store_out<OT>(src.out, o * n + i, acc[v] * src.scale);
+            for (int v = 0; v < 4; ++v) {
+                const int i = T + 16 * qb + 4 * v + kq;
+                if (i >= first && i <= last && (all_ok || lti_ok[i - i0])) {
+                    if (inner)
+                        inner_add<OT>(tc, src, o, i, acc[v], a);
+                    else
+                        store_out<OT>(src.out, o * n + i, acc[v] * src.scale);
+                }
+            }
         }
+        if (inner) inner_commit(*ish, o, a);  // the whole wave, after the per-lane branch
     }
 }
 
@@ -197,8 +293,10 @@ __device__ __forceinline__ void lti_mfma(const FusedSource &src, int first, int 
 template <typename OT, int TPB>
 __device__ __forceinline__ void utv_mfma(const TdiConfig &tc, const FusedSource &src, int first,
                                          int last, int m, int yw_lo, int yw_len, int kmin,
-                                         const double *ywin, int pitch, const double *filt)
+                                         const double *ywin, int pitch, const double *filt,
+                                         InnerShared *ish)
 {
+    const bool inner = ish != nullptr;  // (inner-product kernels only)
     typedef double d4 __attribute__((ext_vector_type(4)));
     const int T = src.tdi_start;
     const int q0 = (first - T) >> 4, q1 = (last - T) >> 4;
@@ -232,11 +330,12 @@ __device__ __forceinline__ void utv_mfma(const TdiConfig &tc, const FusedSource 
                 __asm__ volatile("" ::: "memory");  // one link's operands in flight at a time
             });
         }
-        if (qb > q1) continue;
+        InnerAcc ia;
+        inner_zero(ia);
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int i = T + 16 * qb + 4 * v + kq;
-            if (i < first || i > last) continue;
+            if (qb > q1 || i < first || i > last) continue;
             const double di = (double)(i - m);
             const double X = __builtin_fma(di, a1[0][v], a0[0][v]);
             const double Y = __builtin_fma(di, a1[1][v], a0[1][v]);
@@ -247,10 +346,17 @@ __device__ __forceinline__ void utv_mfma(const TdiConfig &tc, const FusedSource 
                 o1 = (X - 2.0 * Y + Z) * kInvSqrt6;
                 o2 = (X + Y + Z) * kInvSqrt3;
             }
-            store_out<OT>(src.out, i, o0 * src.scale);
-            store_out<OT>(src.out, n + i, o1 * src.scale);
-            store_out<OT>(src.out, 2 * n + i, o2 * src.scale);
+            if (inner) {
+                inner_add<OT>(tc, src, 0, i, o0, ia.a[0]);
+                inner_add<OT>(tc, src, 1, i, o1, ia.a[1]);
+                inner_add<OT>(tc, src, 2, i, o2, ia.a[2]);
+            } else {
+                store_out<OT>(src.out, i, o0 * src.scale);
+                store_out<OT>(src.out, n + i, o1 * src.scale);
+                store_out<OT>(src.out, 2 * n + i, o2 * src.scale);
+            }
         }
+        if (inner) inner_commit_all(*ish, ia);
     }
 }
 
@@ -263,7 +369,8 @@ template <bool EXACT, int MODE, int NX, typename HT, typename OT, int PASSES, in
 __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig &tc,
                                            const FusedSource &src, int s, int tile,
                                            const OrbitView &orb, const LagrangeView &lg,
-                                           const RowMap &rm, double *smem, const double *s_E)
+                                           const RowMap &rm, double *smem, const double *s_E,
+                                           InnerShared *ish = nullptr)
 {
     constexpr int H = 13;
     constexpr int kPasses = PASSES;
@@ -274,7 +381,11 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
     const int i0 = (tc.zero_edges ? 0 : T) + tile * tile_len;
     const int first = max(i0, T);
     const int last = min(i0 + tile_len, n - T) - 1;
-    if (tc.zero_edges) {
+    // an inner-product kernel (composite-filter form only): sums instead of stores
+    const bool inner = ish != nullptr;
+    InnerAcc ia;
+    inner_zero(ia);
+    if (tc.zero_edges && !inner) {
 #pragma unroll
         for (int r = 0; r < kPasses; ++r) {
             const int i = i0 + r * TPB + tid;
@@ -292,8 +403,11 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
             const int i = i0 + r * TPB + tid;
             if (i < first || i > last) continue;
             atomicOr(tc.err, kErrWindow);
-            for (int c = 0; c < 3; ++c) store_out<OT>(src.out, (size_t)c * n + i, __builtin_nan(""));
+            if (inner) ia.a[0][0] = __builtin_nan("");
+            else
+                for (int c = 0; c < 3; ++c) store_out<OT>(src.out, (size_t)c * n + i, __builtin_nan(""));
         }
+        if (inner) inner_commit_all(*ish, ia);
         return;
     }
     // Mixed precision, equal arms: the stock TDI is two composite time-invariant filters per
@@ -363,8 +477,10 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
             for (int r = 0; r < kPasses; ++r) {
                 const int i = i0 + r * TPB + tid;
                 if (i < first || i > last) continue;
-                for (int c = 0; c < 3; ++c)
-                    store_out<OT>(src.out, (size_t)c * n + i, __builtin_nan(""));
+                if (inner) ia.a[0][0] = __builtin_nan("");
+                else
+                    for (int c = 0; c < 3; ++c)
+                        store_out<OT>(src.out, (size_t)c * n + i, __builtin_nan(""));
             }
             if (tid == 0) atomicOr(tc.err, kErrLti);
             lti = false;  // no matrix-core pass either
@@ -388,7 +504,7 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
                 if (ok) continue;
                 bad |= !tdi_linear_padded<NX, true>(t, L, t0_offset, tc.fs, ywin, pitch, yw_lo,
                                                     yw_len, lg, acc);
-                fused_store<OT>(tc, src, i, acc, false);
+                fused_store<OT>(tc, src, i, acc, false, inner ? &ia : nullptr);
             }
             continue;
         }
@@ -426,10 +542,11 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
         fused_store<OT>(tc, src, i, acc, MODE != kModeLinear && tc.aet);
     }
     if constexpr (kLti) {
+        if (inner) inner_commit_all(*ish, ia);  // the per-sample samples' sums
         if (lti) {
             __syncthreads();  // lti_ok written
-            lti_mfma<OT, TPB>(src, first, last, i0, yw_lo, yw_len, lref.kmin, ywin, pitch, fpad, lti_ok,
-                         tile_ok);
+            lti_mfma<OT, TPB>(tc, src, first, last, i0, yw_lo, yw_len, lref.kmin, ywin, pitch,
+                              fpad, lti_ok, tile_ok, ish);
         }
     }
     if (bad) atomicOr(tc.err, kErrWindow);
@@ -452,15 +569,27 @@ void fused_batch_kernel(ProjConfig pc, TdiConfig tc,
 // its VGPRs and the matrix accumulator share one 512-entry register file per lane.
 // TPB threads per workgroup: 256, or 512 for the 1024-sample window (two rounds of 512 lanes;
 // its 72 KB of LDS then still leave 16 waves per CU, with half the halo recompute of 512).
-template <int NX, typename HT, int PASSES, int TPB, bool SAMPLE_PASS>
+// INNER: the inner-product kernel (tc.inner_part set) -- sums against the data series instead
+// of stores.
+template <int NX, typename HT, int PASSES, int TPB, bool SAMPLE_PASS, bool INNER>
 __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4)))
 void fused_lti_kernel(ProjConfig pc, TdiConfig tc, Batch<FusedSource> batch, OrbitView orb,
                       LagrangeView lg, RowMap rm)
 {
     extern __shared__ double smem[];
     __shared__ double s_E[kMaxHalf];
+    InnerShared *ish = nullptr;
+    if constexpr (INNER) {
+        __shared__ InnerShared ish_slots;
+        ish = &ish_slots;
+        inner_clear(*ish);  // visible after the loop's first barrier
+    }
     load_E_lds(lg, s_E);
-    FLR_BATCH_LOOP(batch, (fused_tile<false, kModeLinear, NX, HT, float, PASSES, TPB, SAMPLE_PASS>(pc, tc, src, s_, tile, orb, lg, rm, smem, s_E)));
+    FLR_BATCH_LOOP(batch, {
+        fused_tile<false, kModeLinear, NX, HT, float, PASSES, TPB, SAMPLE_PASS>(
+            pc, tc, src, s_, tile, orb, lg, rm, smem, s_E, ish);
+        if constexpr (INNER) inner_flush(tc, *ish, g);
+    });
 }
 
 // The mixed form with unequal stock arms (order 25): fused_tile's staging and projection (link
@@ -474,7 +603,7 @@ __device__ __forceinline__ void fused_tile_utv(const ProjConfig &pc, const TdiCo
                                                const FusedSource &src, int tile,
                                                const OrbitView &orb, const LagrangeView &lg,
                                                const RowMap &rm, double *smem, const double *s_E,
-                                               UtvShared &sh)
+                                               UtvShared &sh, InnerShared *ish)
 {
     constexpr int H = 13;
     const int tile_len = tc.tile;
@@ -484,7 +613,10 @@ __device__ __forceinline__ void fused_tile_utv(const ProjConfig &pc, const TdiCo
     const int i0 = (tc.zero_edges ? 0 : T) + tile * tile_len;
     const int first = max(i0, T);
     const int last = min(i0 + tile_len, n - T) - 1;
-    if (tc.zero_edges) {
+    const bool inner = ish != nullptr;  // (inner-product kernels only)
+    InnerAcc ia;
+    inner_zero(ia);
+    if (tc.zero_edges && !inner) {
 #pragma unroll
         for (int r = 0; r < PASSES; ++r) {
             const int i = i0 + r * TPB + tid;
@@ -501,8 +633,11 @@ __device__ __forceinline__ void fused_tile_utv(const ProjConfig &pc, const TdiCo
             const int i = i0 + r * TPB + tid;
             if (i < first || i > last) continue;
             atomicOr(tc.err, kErrWindow);
-            for (int c = 0; c < 3; ++c) store_out<float>(src.out, (size_t)c * n + i, __builtin_nan(""));
+            if (inner) ia.a[0][0] = __builtin_nan("");
+            else
+                for (int c = 0; c < 3; ++c) store_out<float>(src.out, (size_t)c * n + i, __builtin_nan(""));
         }
+        if (inner) inner_commit_all(*ish, ia);
         return;
     }
     const int pitch = lti_pitch(cap);
@@ -556,7 +691,7 @@ __device__ __forceinline__ void fused_tile_utv(const ProjConfig &pc, const TdiCo
             filt[kUtvPairs * kFpad + x] = f1;
         }
         __syncthreads();
-        utv_mfma<float, TPB>(tc, src, first, last, m, yw_lo, yw_len, kmin, ywin, pitch, filt);
+        utv_mfma<float, TPB>(tc, src, first, last, m, yw_lo, yw_len, kmin, ywin, pitch, filt, ish);
     } else {
 #pragma unroll 1
         for (int r = 0; r < PASSES; ++r) {
@@ -570,13 +705,14 @@ __device__ __forceinline__ void fused_tile_utv(const ProjConfig &pc, const TdiCo
             double acc[3] = {0.0, 0.0, 0.0};
             bad |= !tdi_stock_unequal_padded<NX>(t, L, t0_offset, tc.fs, ywin, pitch, yw_lo, yw_len,
                                                  lg, acc);
-            fused_store<float>(tc, src, i, acc, tc.aet);
+            fused_store<float>(tc, src, i, acc, tc.aet, inner ? &ia : nullptr);
         }
+        if (inner) inner_commit_all(*ish, ia);
     }
     if (bad) atomicOr(tc.err, kErrWindow);
 }
 
-template <int NX, typename HT, int PASSES, int TPB>
+template <int NX, typename HT, int PASSES, int TPB, bool INNER>
 __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4)))
 void fused_utv_kernel(ProjConfig pc, TdiConfig tc, Batch<FusedSource> batch, OrbitView orb,
                       LagrangeView lg, RowMap rm)
@@ -584,8 +720,17 @@ void fused_utv_kernel(ProjConfig pc, TdiConfig tc, Batch<FusedSource> batch, Orb
     extern __shared__ double smem[];
     __shared__ double s_E[kMaxHalf];
     __shared__ UtvShared sh;
+    InnerShared *ish = nullptr;
+    if constexpr (INNER) {
+        __shared__ InnerShared ish_slots;
+        ish = &ish_slots;
+        inner_clear(*ish);
+    }
     load_E_lds(lg, s_E);
-    FLR_BATCH_LOOP(batch, (fused_tile_utv<NX, HT, PASSES, TPB>(pc, tc, src, tile, orb, lg, rm, smem, s_E, sh)));
+    FLR_BATCH_LOOP(batch, {
+        fused_tile_utv<NX, HT, PASSES, TPB>(pc, tc, src, tile, orb, lg, rm, smem, s_E, sh, ish);
+        if constexpr (INNER) inner_flush(tc, *ish, g);
+    });
 }
 
 // Launch one fused variant over a batch: the TDI mode follows the table (stock + equal arms:
@@ -610,10 +755,16 @@ hipError_t launch_fused_variant(const ProjConfig &pc, const TdiConfig &tc, const
             constexpr int TPB = PASSES == 4 ? 2 * kBlock : kBlock;
             constexpr int P = PASSES == 4 ? 2 : PASSES;
             // without the per-sample pass when the host proved every tile clears as a whole
-#define FLR_LTI(NXV, SP) \
-    launch_tpb(fused_lti_kernel<NXV, HT, P, TPB, SP>, nb, TPB, shm, stream, pc, tc, b, orb, lg, rm)
-            if (st.generation == 1) return tc.lti_uniform ? FLR_LTI(8, false) : FLR_LTI(8, true);
-            return tc.lti_uniform ? FLR_LTI(16, false) : FLR_LTI(16, true);
+#define FLR_LTI(NXV, SP, IN) \
+    launch_tpb(fused_lti_kernel<NXV, HT, P, TPB, SP, IN>, nb, TPB, shm, stream, pc, tc, b, orb, lg, rm)
+            if (tc.inner_part) {
+                if (st.generation == 1)
+                    return tc.lti_uniform ? FLR_LTI(8, false, true) : FLR_LTI(8, true, true);
+                return tc.lti_uniform ? FLR_LTI(16, false, true) : FLR_LTI(16, true, true);
+            }
+            if (st.generation == 1)
+                return tc.lti_uniform ? FLR_LTI(8, false, false) : FLR_LTI(8, true, false);
+            return tc.lti_uniform ? FLR_LTI(16, false, false) : FLR_LTI(16, true, false);
 #undef FLR_LTI
         } else {
             return st.generation == 1 ? FLR_FUSED(kEq, 8) : FLR_FUSED(kEq, 16);
@@ -625,9 +776,10 @@ hipError_t launch_fused_variant(const ProjConfig &pc, const TdiConfig &tc, const
             constexpr int TPB = PASSES == 4 ? 2 * kBlock : kBlock;
             constexpr int P = PASSES == 4 ? 2 : PASSES;
             const size_t shu = fused_lds_bytes(pc, tc, false, false, PASSES, true);
-#define FLR_UTV(NXV) \
-    launch_tpb(fused_utv_kernel<NXV, HT, P, TPB>, nb, TPB, shu, stream, pc, tc, b, orb, lg, rm)
-            return st.generation == 1 ? FLR_UTV(8) : FLR_UTV(16);
+#define FLR_UTV(NXV, IN) \
+    launch_tpb(fused_utv_kernel<NXV, HT, P, TPB, IN>, nb, TPB, shu, stream, pc, tc, b, orb, lg, rm)
+            if (tc.inner_part) return st.generation == 1 ? FLR_UTV(8, true) : FLR_UTV(16, true);
+            return st.generation == 1 ? FLR_UTV(8, false) : FLR_UTV(16, false);
 #undef FLR_UTV
         }
         return st.generation == 1 ? FLR_FUSED(kModeStock, 8) : FLR_FUSED(kModeStock, 16);

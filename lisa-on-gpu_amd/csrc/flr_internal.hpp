@@ -93,7 +93,17 @@ struct TdiConfig {
     int lti_uniform = 0;        // mixed fused launches: the host proved that every tile of every
                                 // source clears the composite-filter test as a whole, so the
                                 // kernel without the per-sample pass runs
+    // Inner-product launches of mixed fused batches (inner_part set): instead of storing the
+    // outputs, every tile writes its per-channel sums against the data series (same layout as
+    // the outputs, at inner_data + (src.out - inner_out)) to inner_part [tile][3][kInnerSums].
+    double *inner_part = nullptr;
+    const char *inner_out = nullptr;
+    const char *inner_data = nullptr;
 };
+
+// Sums of one channel over a tile in an inner-product launch: d h, h h, h and (-1)^(i - T) h
+// (h the output before float32 rounding, d the data series).
+constexpr int kInnerSums = 4;
 
 struct TdiSource {
     double *out;            // [3][num_delays]
@@ -209,6 +219,10 @@ hipError_t launch_fused_c128(const ProjConfig &pc, const TdiConfig &tc,
 // mixed unequal-arm form (h window or filters, padded rows).
 size_t fused_lds_bytes(const ProjConfig &pc, const TdiConfig &tc, bool generic, bool lti,
                        int passes, bool utv = false);
+// Per-source totals of an inner-product launch's tile sums, in a fixed order:
+// sums[s][3][kInnerSums] (flr_kernels.hip).
+hipError_t launch_inner_reduce(const double *part, const int *tile_start, int num_sources,
+                               double *sums, hipStream_t stream);
 // Galactic-binary generator (flr_waveform.hip).
 hipError_t launch_gb(const flr_gb_source *sources, int num_sources, int64_t total, double dt,
                      int format, void *out, hipStream_t stream);

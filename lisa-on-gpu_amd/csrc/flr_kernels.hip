@@ -404,6 +404,41 @@ size_t fused_lds_bytes(const ProjConfig &pc, const TdiConfig &tc, bool generic, 
            (lti ? (size_t)passes * kBlock : 0);
 }
 
+// One workgroup per source: each thread adds the tile records g0 + t, g0 + t + kBlock, ... of
+// its source, then a fixed tree over the threads -- the same order whatever ran before.
+__global__ __launch_bounds__(kBlock) void inner_reduce_kernel(const double *part,
+                                                              const int *tile_start,
+                                                              int num_sources, double *sums)
+{
+    constexpr int K = 3 * kInnerSums;
+    __shared__ double red[K][kBlock];
+    const int s = blockIdx.x, t = threadIdx.x;
+    if (s >= num_sources) return;  // whole workgroup
+    const int g0 = tile_start[s], g1 = tile_start[s + 1];
+    double acc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] = 0.0;
+    for (int g = g0 + t; g < g1; g += kBlock)
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[k] += part[(size_t)g * K + k];
+#pragma unroll
+    for (int k = 0; k < K; ++k) red[k][t] = acc[k];
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (t < w)
+#pragma unroll
+            for (int k = 0; k < K; ++k) red[k][t] += red[k][t + w];
+        __syncthreads();
+    }
+    if (t < K) sums[(size_t)s * K + t] = red[t][0];
+}
+
+hipError_t launch_inner_reduce(const double *part, const int *tile_start, int num_sources,
+                               double *sums, hipStream_t stream)
+{
+    return launch(inner_reduce_kernel, num_sources, 0, stream, part, tile_start, num_sources, sums);
+}
+
 hipError_t launch_fused_batch(const ProjConfig &pc, const TdiConfig &tc,
                               const Batch<FusedSource> &b, const OrbitView &orb,
                               const LagrangeView &lg, bool exact, bool mixed, int input,

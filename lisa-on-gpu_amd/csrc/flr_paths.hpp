@@ -216,8 +216,9 @@ __device__ __forceinline__ int batch_source(const Batch<Source> &b, int g)
 }
 
 // Workgroup b of G walks the contiguous tiles [b*T/G, (b+1)*T/G): neighbouring tiles share
-// halo samples, which then stay in the same XCD's L2.
-#define FLR_BATCH_LOOP(batch, body)                                                        \
+// halo samples, which then stay in the same XCD's L2. The body sees the tile's global index g,
+// its source s_ and record src, and its index `tile` within the source.
+#define FLR_BATCH_LOOP(batch, ...)                                                         \
     do {                                                                                   \
         const long long T_ = (batch).total_tiles;                                          \
         const int g0_ = (int)((T_ * blockIdx.x) / gridDim.x);                              \
@@ -227,7 +228,7 @@ __device__ __forceinline__ int batch_source(const Batch<Source> &b, int g)
             const int s_ = batch_source(batch, g);                                         \
             const auto src = (batch).sources[s_];                                          \
             const int tile = g - (batch).tile_start[s_];                                   \
-            body;                                                                          \
+            __VA_ARGS__;                                                                   \
         }                                                                                  \
     } while (0)
 

@@ -55,6 +55,7 @@ SIGNATURES = {
     "flr_fused_reserve": (_ci, [_vp, _ci, _cd, _cd, _ci, ctypes.POINTER(_vp)]),
     "flr_fused_bind": (_ci, [_vp, _vp, _ci, _vp, _vp, _vp]),
     "flr_batch_run": (_ci, [_vp, _vp]),
+    "flr_fused_inner": (_ci, [_vp, _vp, _vp, _vp]),
     "flr_batch_set_launch": (_ci, [_vp, _ci, _ci]),
     "flr_batch_samples": (ctypes.c_int64, [_vp]),
     "flr_batch_destroy": (None, [_vp]),

@@ -325,6 +325,34 @@ class BatchPlan:
                                    self.out_all.data_ptr(),
                                    current_stream() if stream is None else stream))
 
+    def inner_products(self, data: torch.Tensor, sums: Optional[torch.Tensor] = None,
+                       stream: Optional[int] = None) -> torch.Tensor:
+        """Fused inner products of the plan's responses with a resident data series, without
+        writing the responses (flr_fused_inner): `data` is laid out as out_all and has its dtype
+        (float32 for the mixed forms). Returns sums [num_sources, 3, 4] (float64, device): per
+        source and output channel, over [tdi_start, n - tdi_start), sum d h, sum h^2, sum h and
+        sum (-1)^(i - tdi_start) h, with h the response in FP64 before its float32 rounding
+        (utility.inner_from_sums, overlap_from_sums and log_likelihood_from_sums turn them into
+        inner products, get_overlap's overlap and a white-noise log-likelihood). Mixed plans
+        with the stock TDI tables. Enqueued on `stream` (default: the current one); the first
+        call allocates the per-tile buffer, later ones nothing (graph-capturable after it)."""
+        if self.br.precision == "fp64":
+            raise ValueError("fused inner products run in the mixed-precision kernels "
+                             "(precision='mixed-c128', 'mixed' or 'mixed-q31')")
+        if self.br.tdi_chan == "AE":
+            raise ValueError("inner products need all three channels (tdi_chan 'AET' or 'XYZ')")
+        if (not isinstance(data, torch.Tensor) or not data.is_cuda or data.dtype != self.out_all.dtype
+                or data.numel() != self.out_all.numel() or not data.is_contiguous()):
+            raise TypeError("data must be a contiguous device tensor laid out as out_all")
+        n = len(self.lengths)
+        if sums is None:
+            sums = torch.empty((n, 3, 4), dtype=torch.float64, device=self.out_all.device)
+        elif sums.shape != (n, 3, 4) or sums.dtype != torch.float64 or not sums.is_contiguous():
+            raise TypeError("sums must be a contiguous float64 tensor of shape (sources, 3, 4)")
+        check(lib().flr_fused_inner(self._h, data.data_ptr(), sums.data_ptr(),
+                                    current_stream() if stream is None else stream))
+        return sums
+
     def tune(self, **kw) -> dict:
         """Pick the fastest launch shape for this plan (fastlisaresponse_amd.autotune.tune)."""
         from .autotune import tune

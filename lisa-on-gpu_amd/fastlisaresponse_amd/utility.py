@@ -1,4 +1,5 @@
-"""Channel helpers: XYZ -> AET and the FFT overlap used by the reference's tests."""
+"""Channel helpers: XYZ -> AET, the FFT overlap used by the reference's tests, and the
+quantities that follow from the fused inner-product sums (BatchPlan.inner_products)."""
 from __future__ import annotations
 
 from typing import Optional
@@ -63,3 +64,62 @@ def get_overlap(sig1, sig2, phase_maximize: bool = False, use_gpu: Optional[bool
             overlap += np.dot(fa.conj(), fb) / np.sqrt(np.dot(fa.conj(), fa) * np.dot(fb.conj(), fb))
         overlap /= len(s1)
     return float(abs(overlap)) if phase_maximize else float(overlap.real)
+
+
+# ------------------------------------------------------------------ fused inner products
+# BatchPlan.inner_products returns, per source and channel, over the N = n - 2*tdi_start
+# samples of [tdi_start, n - tdi_start): S0 = sum d h, S1 = sum h^2, S2 = sum h and
+# S3 = sum (-1)^j h (j counted from tdi_start). For real series a, b of length N,
+#   sum_{k=0}^{floor(N/2)} conj(rfft(a)_k) rfft(b)_k
+# has real part (N sum a b + a_0 b_0 + [N even] a_N b_N) / 2 (Parseval, with the DC bin
+# a_0 = sum a and the Nyquist bin a_N = sum (-1)^j a counted once), which is how get_overlap's
+# rfft inner products follow from time-domain sums without the series.
+
+
+def data_sums(data, lengths, offsets, tdi_start: int) -> torch.Tensor:
+    """sum d^2, sum d, sum (-1)^j d per source and channel of a data series laid out as a plan's
+    out_all ([sources, 3, 3], float64), over the same samples as the fused sums."""
+    out = []
+    for o, n in zip(offsets, lengths):
+        d = data[3 * int(o): 3 * (int(o) + int(n))].view(3, int(n))[:, tdi_start:int(n) - tdi_start]
+        d = d.to(torch.float64)
+        alt = torch.ones(d.shape[1], dtype=torch.float64, device=d.device)
+        alt[1::2] = -1.0
+        out.append(torch.stack([(d * d).sum(1), d.sum(1), (d * alt).sum(1)], dim=1))
+    return torch.stack(out)
+
+
+def _rfft_products(N, S_ab, a0, b0, aN, bN):
+    """Real part of the rfft inner product of two length-N series from their time-domain
+    product sum and DC / Nyquist bins."""
+    nyq = (aN * bN) if (N % 2 == 0) else 0.0
+    return 0.5 * (N * S_ab + a0 * b0 + nyq)
+
+
+def overlap_from_sums(sums, dsums, N: int) -> torch.Tensor:
+    """get_overlap(data, response) (utils/utility.py:16-82, phase_maximize=False) per source,
+    from the fused sums and data_sums over N samples: the channel average of
+    Re<d, h> / sqrt(<d, d> <h, h>) with <., .> the rfft inner product."""
+    S0, S1, S2, S3 = sums.unbind(-1)
+    D1, D2, D3 = dsums.unbind(-1)
+    dh = _rfft_products(N, S0, D2, S2, D3, S3)
+    hh = _rfft_products(N, S1, S2, S2, S3, S3)
+    dd = _rfft_products(N, D1, D2, D2, D3, D3)
+    return (dh / torch.sqrt(dd * hh)).mean(-1)
+
+
+def inner_from_sums(sums, noise_var=1.0) -> torch.Tensor:
+    """White-noise inner products (d|h) = sum_c sum_i d h / sigma_c^2 and (h|h) per source
+    ([sources, 2]); noise_var: sigma^2, a scalar or one value per channel."""
+    w = torch.as_tensor(noise_var, dtype=torch.float64, device=sums.device).reshape(-1)
+    w = w.expand(3) if w.numel() == 1 else w
+    return torch.stack([(sums[..., 0] / w).sum(-1), (sums[..., 1] / w).sum(-1)], dim=-1)
+
+
+def log_likelihood_from_sums(sums, dsums, noise_var=1.0) -> torch.Tensor:
+    """Gaussian white-noise log-likelihood -1/2 sum_c sum_i (d - h)^2 / sigma_c^2 per source
+    (up to the normalisation constant), from the fused sums and data_sums."""
+    w = torch.as_tensor(noise_var, dtype=torch.float64, device=sums.device).reshape(-1)
+    w = w.expand(3) if w.numel() == 1 else w
+    chi2 = (dsums[..., 0] - 2.0 * sums[..., 0] + sums[..., 1]) / w
+    return -0.5 * chi2.sum(-1)
