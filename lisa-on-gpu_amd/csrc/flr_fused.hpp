@@ -231,14 +231,15 @@ __global__ __launch_bounds__(kBlock) void lti_filter_kernel(TdiConfig tc,
 // which is D = A B with A[r][m] = F[m - r] (Toeplitz, the padded filters) and
 // B[m][block] = the series: one v_mfma_f64_16x16x4f64 per 4 taps of 16 blocks x 16 outputs
 // (28 per group of 16 blocks of one output). Operand layout: lane l holds A[l % 16][l / 16]
-// and B[l / 16][l % 16]; the result's element v is D[4v + l / 16][l % 16]. Tap columns are
-// clamped into the window, so edge blocks read finite samples wherever F is zero; their rows
-// outside the tile, and samples left to the per-sample path (lti_ok = 0 unless all_ok), are
-// not stored.
+// and B[l / 16][l % 16]; the result's element v is D[4v + l / 16][l % 16]. The rows are
+// block-interleaved (blk_col), so each B operand is one load at a per-lane base plus a
+// compile-time offset; edge blocks read the zeroed guard and tail slots wherever F is zero,
+// their rows outside the tile, and samples left to the per-sample path (lti_ok = 0 unless
+// all_ok), are not stored.
 template <typename OT, int TPB>
 __device__ __forceinline__ void lti_mfma(const TdiConfig &tc, const FusedSource &src, int first,
                                          int last, int i0, int yw_lo, int yw_len, int kmin,
-                                         const double *ywin, int pitch, const double *fpad,
+                                         const double *ywin, int pitch, int R, const double *fpad,
                                          const unsigned char *lti_ok, bool all_ok,
                                          InnerShared *ish)
 {
@@ -253,7 +254,13 @@ __device__ __forceinline__ void lti_mfma(const TdiConfig &tc, const FusedSource 
     for (int g = wave; g < 3 * ngc; g += TPB / 64) {
         const int o = g / ngc;
         const int qb = q0 + 16 * (g - o * ngc) + nb;
-        const int wb = T + 16 * min(qb, q1) + kmin - yw_lo;  // window column of tap m = 0
+        // window column of this lane's tap step 0 (tap m = kq of block min(qb, q1)); a block
+        // with a sample that uses the filter starts at column -15 or later (lti_matches), one
+        // whose samples all take the per-sample path is held to the guard
+        const int w0 = max(T + 16 * min(qb, q1) + kmin - yw_lo + kq, -16);
+        int b[4];  // tap step j = 4a + i at b[i] + a
+#pragma unroll
+        for (int i = 0; i < 4; ++i) b[i] = blk_col(w0 + 4 * i, R);
         d4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int kind = 0; kind < 2; ++kind) {
@@ -261,8 +268,8 @@ __device__ __forceinline__ void lti_mfma(const TdiConfig &tc, const FusedSource 
             const double *f = fpad + kind * kFpad + kFpadLo + kq - nb;  // f[4j] = F[4j + kq - nb]
 #pragma unroll
             for (int j = 0; j < 14; ++j) {
-                const int w = min(max(wb + 4 * j + kq, 0), yw_len - 1);
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(f[4 * j], row[lti_col(w)], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(f[4 * j], row[b[j & 3] + (j >> 2)], acc,
+                                                          0, 0, 0);
                 __asm__ volatile("" ::: "memory");  // one operand pair in flight: no register growth
             }
         }
@@ -293,7 +300,7 @@ __device__ __forceinline__ void lti_mfma(const TdiConfig &tc, const FusedSource 
 template <typename OT, int TPB>
 __device__ __forceinline__ void utv_mfma(const TdiConfig &tc, const FusedSource &src, int first,
                                          int last, int m, int yw_lo, int yw_len, int kmin,
-                                         const double *ywin, int pitch, const double *filt,
+                                         const double *ywin, int pitch, int R, const double *filt,
                                          InnerShared *ish)
 {
     const bool inner = ish != nullptr;  // (inner-product kernels only)
@@ -306,14 +313,18 @@ __device__ __forceinline__ void utv_mfma(const TdiConfig &tc, const FusedSource 
     const size_t n = (size_t)src.num_pts;
     for (int g = wave; g < ngc; g += TPB / 64) {
         const int qb = q0 + 16 * g + nb;
-        const int wb = T + 16 * min(qb, q1) + kmin - yw_lo;
+        // window column of this lane's tap step 0 (>= -15: the tile's taps fit its window)
+        const int w0 = T + 16 * min(qb, q1) + kmin - yw_lo + kq;
+        int b[4];  // tap step j = 4a + i at b[i] + a
+#pragma unroll
+        for (int i = 0; i < 4; ++i) b[i] = blk_col(w0 + 4 * i, R);
         d4 a0[3], a1[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) a0[c] = a1[c] = d4{0.0, 0.0, 0.0, 0.0};
         const double *fl = filt + kFpadLo + kq - nb;  // fl[p * kFpad + 4j] = F0_p[4j + kq - nb]
 #pragma unroll
         for (int j = 0; j < 14; ++j) {
-            const int col = lti_col(min(max(wb + 4 * j + kq, 0), yw_len - 1));
+            const int col = b[j & 3] + (j >> 2);
             static_for<6>([&](auto l) {  // canonical link l: one operand for its two pairs
                 const double yv = ywin[l * pitch + col];
                 static_for<3>([&](auto c) {
@@ -413,9 +424,10 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
     // Mixed precision, equal arms: the stock TDI is two composite time-invariant filters per
     // output (built once per source by lti_filter_kernel, referenced at the source's first TDI
     // sample so that every tile applies the same filter) applied on the matrix cores; the
-    // window rows are then padded (lti_col) for the matrix operands' loads.
+    // window rows are then block-interleaved (blk_col) for the matrix operands' loads.
     constexpr bool kLti = MODE == kModeLinear && std::is_same<OT, float>::value;
-    const int pitch = kLti ? lti_pitch(cap) : cap;
+    const int R = blk_R(cap);
+    const int pitch = kLti ? blk_pitch(cap) : cap;
     const int hw_lo = max(0, yw_lo - pc.halo_lo);
     const int hw_len = min(n, yw_lo + yw_len + pc.halo_hi) - hw_lo;  // <= pc.win_cap (host-sized)
     double2 *hwin = reinterpret_cast<double2 *>(smem);
@@ -423,6 +435,7 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
     double *fpad = ywin + kNLinks * pitch;  // kLti: [2][kFpad]
     unsigned char *lti_ok = reinterpret_cast<unsigned char *>(fpad + 2 * kFpad);  // [PASSES * TPB]
     for (int jj = tid; jj < hw_len; jj += TPB) hwin[jj] = load_h<HT>(src.h, hw_lo + jj);
+    if constexpr (kLti) blk_zero(ywin, pitch, R, yw_len, tid, TPB);  // before the window's stores
     __syncthreads();
 
     const double t0_offset = src.t0 + (double)0 * pc.dt;
@@ -430,7 +443,7 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
     bool bad = false;
     for (int w = tid; w < yw_len; w += TPB) {
         const int j = yw_lo + w;
-        int col = kLti ? lti_col(w) : w;
+        int col = kLti ? blk_col(w, R) : w;
         pin(col);  // an opaque column: the padded index is not re-derived inside the projection
         double *yc = ywin + col;
         if (j < src.proj_start || j >= n - src.proj_start) {
@@ -502,8 +515,8 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
                     lti && lti_matches<NX>(t, i, L, t0_offset, tc.fs, lref, yw_lo, yw_len);
                 lti_ok[i - i0] = ok;
                 if (ok) continue;
-                bad |= !tdi_linear_padded<NX, true>(t, L, t0_offset, tc.fs, ywin, pitch, yw_lo,
-                                                    yw_len, lg, acc);
+                bad |= !tdi_linear_padded<NX, true>(t, L, t0_offset, tc.fs, ywin, pitch, R,
+                                                    yw_lo, yw_len, lg, acc);
                 fused_store<OT>(tc, src, i, acc, false, inner ? &ia : nullptr);
             }
             continue;
@@ -545,7 +558,7 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
         if (inner) inner_commit_all(*ish, ia);  // the per-sample samples' sums
         if (lti) {
             __syncthreads();  // lti_ok written
-            lti_mfma<OT, TPB>(tc, src, first, last, i0, yw_lo, yw_len, lref.kmin, ywin, pitch,
+            lti_mfma<OT, TPB>(tc, src, first, last, i0, yw_lo, yw_len, lref.kmin, ywin, pitch, R,
                               fpad, lti_ok, tile_ok, ish);
         }
     }
@@ -640,7 +653,8 @@ __device__ __forceinline__ void fused_tile_utv(const ProjConfig &pc, const TdiCo
         if (inner) inner_commit_all(*ish, ia);
         return;
     }
-    const int pitch = lti_pitch(cap);
+    const int R = blk_R(cap);
+    const int pitch = blk_pitch(cap);
     const int hw_lo = max(0, yw_lo - pc.halo_lo);
     const int hw_len = min(n, yw_lo + yw_len + pc.halo_hi) - hw_lo;
     double2 *hwin = reinterpret_cast<double2 *>(smem);
@@ -657,13 +671,14 @@ __device__ __forceinline__ void fused_tile_utv(const ProjConfig &pc, const TdiCo
         sh.ok = 1;
     }
     for (int jj = tid; jj < hw_len; jj += TPB) hwin[jj] = load_h<HT>(src.h, hw_lo + jj);
+    blk_zero(ywin, pitch, R, yw_len, tid, TPB);  // before the window's stores
     __syncthreads();
     if (tid < 3 * NX) utv_unit<NX>(sh, tid, first, m, last, t0_offset, tc.fs, lg);
 
     bool bad = false;
     for (int w = tid; w < yw_len; w += TPB) {
         const int j = yw_lo + w;
-        int col = lti_col(w);
+        int col = blk_col(w, R);
         pin(col);
         double *yc = ywin + col;
         if (j < src.proj_start || j >= n - src.proj_start) {
@@ -691,7 +706,8 @@ __device__ __forceinline__ void fused_tile_utv(const ProjConfig &pc, const TdiCo
             filt[kUtvPairs * kFpad + x] = f1;
         }
         __syncthreads();
-        utv_mfma<float, TPB>(tc, src, first, last, m, yw_lo, yw_len, kmin, ywin, pitch, filt, ish);
+        utv_mfma<float, TPB>(tc, src, first, last, m, yw_lo, yw_len, kmin, ywin, pitch, R, filt,
+                             ish);
     } else {
 #pragma unroll 1
         for (int r = 0; r < PASSES; ++r) {
@@ -703,8 +719,8 @@ __device__ __forceinline__ void fused_tile_utv(const ProjConfig &pc, const TdiCo
 #pragma unroll
             for (int q = 0; q < kNLinks; ++q) L[q] = orbit_val(nd, rm.row[q]);
             double acc[3] = {0.0, 0.0, 0.0};
-            bad |= !tdi_stock_unequal_padded<NX>(t, L, t0_offset, tc.fs, ywin, pitch, yw_lo, yw_len,
-                                                 lg, acc);
+            bad |= !tdi_stock_unequal_padded<NX>(t, L, t0_offset, tc.fs, ywin, pitch, R, yw_lo,
+                                                 yw_len, lg, acc);
             fused_store<float>(tc, src, i, acc, tc.aet, inner ? &ia : nullptr);
         }
         if (inner) inner_commit_all(*ish, ia);

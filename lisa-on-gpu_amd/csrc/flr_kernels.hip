@@ -395,10 +395,10 @@ size_t fused_lds_bytes(const ProjConfig &pc, const TdiConfig &tc, bool generic, 
 {
     if (utv)  // h window or the filters, then the padded rows (fused_tile_utv)
         return ((size_t)std::max(2 * pc.win_cap, 2 * kUtvPairs * kFpad) +
-                (size_t)kNLinks * lti_pitch(tc.win_cap)) *
+                (size_t)kNLinks * blk_pitch(tc.win_cap)) *
                sizeof(double);
     const size_t doubles =
-        lti ? (size_t)kNLinks * lti_pitch(tc.win_cap) + 2 * kFpad
+        lti ? (size_t)kNLinks * blk_pitch(tc.win_cap) + 2 * kFpad
             : (size_t)kNLinks * tc.win_cap + (generic ? (size_t)kNLinks * kBlock : 0);
     return (size_t)pc.win_cap * sizeof(double2) + doubles * sizeof(double) +
            (lti ? (size_t)passes * kBlock : 0);

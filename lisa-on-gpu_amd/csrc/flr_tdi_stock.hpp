@@ -400,29 +400,50 @@ __device__ __forceinline__ bool tdi_linear(double t, double L, double t0_offset,
                                acc);
 }
 
-// Column w of a padded window row lives at w + w / 16 (one spare double per 16 samples), so that
-// 16 lanes reading columns 16 apart hit 16 different LDS bank pairs; rows have pitch
-// lti_pitch(cap).
-__host__ __device__ constexpr int lti_col(int w) { return w + (w >> 4); }
-__host__ __device__ constexpr int lti_pitch(int cap) { return cap + (cap >> 4) + 1; }
+// Rows of the matrix-core forms, block-interleaved: window column w is stored at
+//   blk_col(w, R) = (w & 15) * R + (w >> 4) + 1
+// in a row of 16 R doubles (16 phases of R slots; slot 0 of every phase is a zero guard for
+// columns -16 .. -1, and slots past the window are zero too). Columns 16 apart are adjacent
+// doubles, so a matrix operand's 16 block columns are 16 consecutive doubles (no bank
+// conflicts), and its tap steps j = 4a + i sit at blk_col(w0 + 4i) + a: four per-lane bases
+// and compile-time offsets. R (odd, so that a wave storing consecutive columns spreads over
+// the banks) leaves room for the operands' reach of up to 56 columns past a block start.
+__host__ __device__ constexpr int blk_R(int cap) { return ((cap + 79) >> 4) | 1; }
+__host__ __device__ constexpr int blk_pitch(int cap) { return 16 * blk_R(cap); }
+__host__ __device__ constexpr int blk_col(int w, int R) { return (w & 15) * R + (w >> 4) + 1; }
 
-// Padded window column c0 + k of a row, read through operator[].
-struct PaddedTaps {
+// Zero the guard slots and the slots past a window of len columns in the six rows at ywin
+// (before the projection fills the window, which overwrites the last partial slot).
+__device__ __forceinline__ void blk_zero(double *ywin, int pitch, int R, int len, int tid,
+                                         int nthreads)
+{
+    const int s0 = 1 + (len >> 4);  // first slot that may hold no window column
+    const int per = R - s0 + 1;     // slot 0 and slots s0 .. R-1, per phase
+    for (int x = tid; x < kNLinks * 16 * per; x += nthreads) {
+        const int r = x / (16 * per), rem = x - r * 16 * per;
+        const int ph = rem / per, k = rem - ph * per;
+        ywin[r * pitch + ph * R + (k == 0 ? 0 : s0 + k - 1)] = 0.0;
+    }
+}
+
+// Block-interleaved window column c0 + k of a row, read through operator[].
+struct BlockTaps {
     const double *row;
+    int R;
     int c0;
-    __device__ __forceinline__ double operator[](int k) const { return row[lti_col(c0 + k)]; }
+    __device__ __forceinline__ double operator[](int k) const { return row[blk_col(c0 + k, R)]; }
 };
 
-// tdi_linear on padded rows (the per-sample fallback of the matrix-core path).
+// tdi_linear on block-interleaved rows (the per-sample fallback of the matrix-core path).
 template <int NX, bool CE = false>
 __device__ __forceinline__ bool tdi_linear_padded(double t, double L, double t0_offset, double fs,
-                                                  const double *ywin, int pitch, int win_lo,
-                                                  int win_len, const LagrangeView &lg,
+                                                  const double *ywin, int pitch, int R,
+                                                  int win_lo, int win_len, const LagrangeView &lg,
                                                   double (&acc)[3])
 {
     return tdi_linear_cols<NX, CE>(t, L, t0_offset, fs, win_lo, win_len, lg,
                                [&](int o, int kind, int c0) {
-                                   return PaddedTaps{ywin + (2 * o + kind) * pitch, c0};
+                                   return BlockTaps{ywin + (2 * o + kind) * pitch, R, c0};
                                },
                                acc);
 }
@@ -736,19 +757,12 @@ __device__ __forceinline__ void utv_filter_entry(const UtvShared &sh, const Lagr
     }
 }
 
-// Padded window column c + k of a row (c: the column of the stencil's sample d).
-struct PaddedAt {
-    const double *row;
-    int c;
-    __device__ __forceinline__ double operator[](int k) const { return row[lti_col(c + k)]; }
-};
-
-// tdi_stock_unequal (fast stencils) on padded rows in canonical link order (row k at ywin +
-// k * pitch): the per-sample path of the slowly-varying-arm form.
+// tdi_stock_unequal (fast stencils) on block-interleaved rows in canonical link order (row k at
+// ywin + k * pitch): the per-sample path of the slowly-varying-arm form.
 template <int NX>
 __device__ __forceinline__ bool tdi_stock_unequal_padded(double t, const double (&L)[6],
                                                          double t0_offset, double fs,
-                                                         const double *ywin, int pitch,
+                                                         const double *ywin, int pitch, int R,
                                                          int win_lo, int win_len,
                                                          const LagrangeView &lg, double (&acc)[3])
 {
@@ -764,7 +778,8 @@ __device__ __forceinline__ bool tdi_stock_unequal_padded(double t, const double 
             split_delay(delay, t0_offset, fs, d, e);
             const bool bad = (d - win_lo - (H - 1) < 0) || (d - win_lo + H >= win_len);
             ok = ok && !bad;
-            const PaddedAt at{ywin + stock::base_of(c, x) * pitch, bad ? H - 1 : d - win_lo};
+            // at[k] = window column of sample d + k
+            const BlockTaps at{ywin + stock::base_of(c, x) * pitch, R, bad ? H - 1 : d - win_lo};
             double val = stencil_fast<H>(at, e, lg);
             val = bad ? __builtin_nan("") : val;
             val *= (double)stock::kX[x].sign;
