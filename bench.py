@@ -381,10 +381,12 @@ def main() -> int:
                      "autotune_ms": tune["ms"]}
         wall, ms = timed(plan.run, args.steps, args.warmup)
         plan.check_errors()
+        plans[precision] = plan
         return ({"value": N_SAMPLES * args.steps / wall, "ms_per_step": wall * 1e3 / args.steps,
                  "kernel_ms": ms, "launch_shape": shape, "form": FORMS[precision]},
                 plan.out_all.view(3, -1))
 
+    plans = {}
     outs = {}
     out_fp64 = None
     if args.only in (None, "mixed"):
@@ -489,6 +491,84 @@ def main() -> int:
         if "value" in line:
             line["mixed_vs_fp64_fast"] = line["value"] / line["fp64_fast"]["value"]
             line["mixed_vs_fp64_fused"] = line["value"] / line["fp64_fused"]["value"]
+
+    if args.only is None and args.precision in plans:
+        # fused inner products against a resident data series (here the response itself,
+        # stored as float32): the same launch summing d h, h^2, h and the alternating sum per
+        # channel instead of writing A, E, T; beside it the store-then-reduce alternative
+        plan = plans[args.precision]
+        data = plan.out_all.clone()
+        sums = torch.empty((1, 3, 4), dtype=torch.float64, device="cuda")
+        warm_up(lambda: plan.inner_products(data, sums))
+        wi, msi = timed(lambda: plan.inner_products(data, sums), args.steps, args.warmup)
+        plan.check_errors()
+        dd = data.view(3, -1).double()
+
+        def store_then_dot():
+            plan.run()
+            o = plan.out_all.view(3, -1).double()
+            return (o * dd).sum(1), (o * o).sum(1)
+
+        warm_up(store_then_dot)
+        ws, _ = timed(store_then_dot, args.steps, args.warmup)
+        T0 = int(T_BUFFER / DT)
+        hh = dd[:, T0:N_SAMPLES - T0]
+        got = sums[0].cpu().numpy()
+        line["inner_products"] = {
+            "value": N_SAMPLES * args.steps / wi, "ms_per_step": wi * 1e3 / args.steps,
+            "kernel_ms": msi, "store_then_torch_dot_ms_per_step": ws * 1e3 / args.steps,
+            "rel_err_sum_hh_vs_float32_outputs": [
+                float(abs(got[c, 1] - float((hh[c] * hh[c]).sum())) / got[c, 1]) for c in range(3)],
+            "form": "BatchPlan.inner_products (flr_fused_inner): sums of d h, h^2, h and "
+                    "(-1)^j h per channel in the fused launch, no A, E, T written"}
+
+    if args.only is None:
+        # unequal, time-varying arms (CartwheelOrbits): the mixed-c128 fused launch (composite
+        # filters per tile with a drift term, on the matrix cores) against the FP64 drop-in
+        orb_u = CartwheelOrbits(t_max=2.0 * YRSID_SI)
+        orb_u.configure(linear_interp_setup=True)
+        bu = BatchedResponse(1 / DT, tdi=args.tdi, orbits=orb_u, tdi_chan="AET",
+                             precision="mixed-c128")
+        pu = bu.plan(h, [N_SAMPLES], [lam], [beta])
+        warm_up(pu.run)
+        tu = pu.tune(use_cache=False, reps=2)
+        wu, msu = timed(pu.run, args.steps, args.warmup)
+        pu.check_errors()
+        ru = pyResponseTDI(1 / DT, N_SAMPLES, order=ORDER, tdi=args.tdi, orbits=orb_u,
+                           tdi_chan="AET")
+        resu = {}
+
+        def fp64u_step():
+            ru.get_projections(h, lam, beta, t0=0.0, t_buffer=T_BUFFER)
+            resu["out"] = ru.get_tdi_delays()
+
+        warm_up(fp64u_step)
+        wu64, _ = timed(fp64u_step, args.steps, args.warmup)
+        ru.check_errors()
+        un = {"mixed_c128": {"value": N_SAMPLES * args.steps / wu,
+                             "ms_per_step": wu * 1e3 / args.steps, "kernel_ms": msu,
+                             "launch_shape": list(tu["best"]),
+                             "achieved_TFLOPs": flop * N_SAMPLES / (msu * 1e-3) / 1e12},
+              "fp64": {"value": N_SAMPLES * args.steps / wu64,
+                       "ms_per_step": wu64 * 1e3 / args.steps,
+                       "form": "plain FP64 drop-in, exact stencils"},
+              "orbits": "CartwheelOrbits: six different light travel times drifting ~1e-10 s/s"}
+        un["mixed_vs_fp64"] = un["mixed_c128"]["value"] / un["fp64"]["value"]
+        if not args.no_parity:
+            from oracle import oracle as orc
+            oou = orc.OracleOrbits(*orb_u.pycppdetector_args)
+            _, ref_u = orc.response_pipeline(oou, h_host, lam, beta, DT, ORDER, args.tdi,
+                                             x_base=orb_u.x_base,
+                                             nthreads=max(1, min(16, os.cpu_count() or 1)))
+            T0 = int(T_BUFFER / DT)
+            for name, out in (("mixed_c128", pu.out_all.view(3, -1)),
+                              ("fp64", torch.stack(resu["out"]))):
+                got = out.double().cpu().numpy()
+                un[name]["rel_l2_AET_vs_oracle"] = [
+                    float(np.linalg.norm(got[c, T0:-T0] - ref_u[c, T0:-T0]) /
+                          np.linalg.norm(ref_u[c, T0:-T0])) for c in range(3)]
+                un[name]["bitwise_equal_oracle"] = bool(np.array_equal(got, ref_u))
+        line["unequal_arms"] = un
 
     if not args.no_parity:
         from oracle import oracle as orc
