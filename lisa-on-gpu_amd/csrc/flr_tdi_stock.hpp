@@ -618,6 +618,7 @@ struct UtvShared {
     double th[3], tl[3];
     double Lh[3][6], Ll[3][6];
     double off[48], e[48], sigma[48];
+    double A[48], dA[48];  // the interpolated A table at e_u and its slope in e
     int kmin, kmax, ok;
 };
 
@@ -695,6 +696,9 @@ __device__ __forceinline__ void utv_unit(UtvShared &sh, int u, int a, int m, int
     sh.off[u] = (double)off_m;
     sh.e[u] = e_m;
     sh.sigma[u] = sigma;
+    const double2 as = lg.Aslope[ind];  // A as lagrange_A_fast interpolates it, and its slope
+    sh.A[u] = __builtin_fma(as.y, s_m - (double)ind, as.x);
+    sh.dA[u] = as.y * lg.inv_deps;
     atomicMin(&sh.kmin, off_m - (H - 1));
     atomicMax(&sh.kmax, off_m + H);
     if (!ok) sh.ok = 0;
@@ -702,16 +706,11 @@ __device__ __forceinline__ void utv_unit(UtvShared &sh, int u, int a, int m, int
 
 // Weight of tap d + r (r = -(H-1) .. H) of the fast stencil at fraction e, and its derivative
 // in e: w_1 = A (1 - e), w_0 = A e, w_{j+1} = A D E_{j-1} / (j + e), w_{-j} = A D E_{j-1} /
-// (j + 1 - e), D = e (1 - e), A the linearly interpolated table (slope within e's segment).
-__device__ __forceinline__ void lagrange_weight_d(const LagrangeView &lg, const double *E_lds,
+// (j + 1 - e), D = e (1 - e); A is the linearly interpolated table at e and dA its slope
+// within e's segment (utv_unit).
+__device__ __forceinline__ void lagrange_weight_d(double A, double dA, const double *E_lds,
                                                   double e, int r, double &w, double &dw)
 {
-    const double s = e * lg.inv_deps;
-    int ind = (int)s;
-    ind = ind < 0 ? 0 : (ind > lg.num_A - 2 ? lg.num_A - 2 : ind);
-    const double2 a = lg.Aslope[ind];
-    const double A = __builtin_fma(a.y, s - (double)ind, a.x);
-    const double dA = a.y * lg.inv_deps;
     const double B = 1.0 - e;
     const double D = e * B;
     const double dD = 1.0 - 2.0 * e;
@@ -724,7 +723,7 @@ __device__ __forceinline__ void lagrange_weight_d(const LagrangeView &lg, const 
     } else {
         const bool up = r > 1;
         const int j = up ? r - 1 : -r;
-        const double inv_q = 1.0 / (up ? (double)j + e : (double)(j + 1) - e);
+        const double inv_q = rcp_nr2(up ? (double)j + e : (double)(j + 1) - e);
         const double Ej = E_lds[j - 1];
         const double AD = A * D;
         w = AD * Ej * inv_q;
@@ -750,7 +749,7 @@ __device__ __forceinline__ void utv_filter_entry(const UtvShared &sh, const Lagr
         const int r = k + sh.kmin - (int)sh.off[u];
         if (r < -(H - 1) || r > H) continue;
         double w, dw;
-        lagrange_weight_d(lg, E_lds, sh.e[u], r, w, dw);
+        lagrange_weight_d(sh.A[u], sh.dA[u], E_lds, sh.e[u], r, w, dw);
         const double sg = (double)stock::kX[x].sign;
         f0 = __builtin_fma(sg, w, f0);
         f1 = __builtin_fma(-sg * sh.sigma[u], dw, f1);
