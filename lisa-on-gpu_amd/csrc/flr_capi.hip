@@ -92,6 +92,8 @@ struct flr_orbits {
     bool equal_arm = false;  // all six light-travel-time columns bitwise equal at every node
     bool constant_arm = false;  // ... and equal at every node (zero slopes): ltt0 everywhere
     double ltt0 = 0.0;
+    double max_rate = 0.0;  // largest |slope| of a light travel time over the table (s/s)
+    double max_kink = 0.0;  // largest |change of that slope| at a node (s/s)
     int device = 0;
 
     int row_of(int link) const
@@ -424,6 +426,29 @@ bool stock_e25(const flr_response *r)
 // of a tile is monotone in its last sample and the orbit nodes it covers, so the whole TDI range
 // [T, n - T - 1] decides for every tile. Needs constant, equal arms (the only case where the
 // composite filters are used throughout).
+// Whether the slowly-varying-arm form (fused_utv_kernel) suits these orbits at tiles of `tile`
+// samples: from the table's largest arm rate and largest slope change at a node, with every
+// unit taken at the worst case, the expected share of tiles that fail utv_unit's tests -- the
+// chord test at a node (3 x the mid-tile deviation of a kink of the chain's summed slope
+// changes), the drift bound, and a crossing of one of the A table's nodes (probability per unit
+// ~ the tile's range of e over the table spacing) -- is below one half. Otherwise the per-sample
+// kernel runs, which is faster than the filter form's per-sample path on its block-interleaved
+// rows. Only the speed depends on this choice.
+bool utv_suitable(const flr_orbits *o, int nx, int tile, double dt, double fs, double deps)
+{
+    const int M = nx == 8 ? 3 : 7;  // longest chain
+    const double per_sample = dt * fs;
+    const double sigma = M * o->max_rate * per_sample;   // position drift per sample
+    const double half = 0.5 * tile;
+    const double drift = sigma * half;
+    if (drift > kUtvDrift) return false;
+    const double kink = M * o->max_kink * per_sample;    // change of that drift at a node
+    if (3.0 * kink * tile / 4.0 > kLtiTol) return false;
+    const double p = std::min(1.0, 2.0 * drift / deps);  // per unit
+    const double fail = 1.0 - std::pow(1.0 - p, 3.0 * nx);
+    return fail < 0.5;
+}
+
 bool lti_source_uniform(const flr_orbits *o, const FusedSource &f, double dt, double fs, int nx,
                         int halo_lo, int halo_hi)
 {
@@ -659,6 +684,14 @@ int flr_orbits_create(double dt, int N, const double *n_arr, const double *ltt_a
     for (int w = 0; w < N && o->equal_arm; ++w)
         for (int r = 1; r < kNLinks; ++r)
             if (table[(size_t)w * kOrbitStride + r] != table[(size_t)w * kOrbitStride]) o->equal_arm = false;
+    for (int w = 0; w + 1 < N; ++w)
+        for (int r = 0; r < kNLinks; ++r) {
+            const double sl = table[(size_t)w * kOrbitStride + kOrbitVals + r];
+            o->max_rate = std::max(o->max_rate, std::fabs(sl));
+            if (w > 0)
+                o->max_kink = std::max(
+                    o->max_kink, std::fabs(sl - table[(size_t)(w - 1) * kOrbitStride + kOrbitVals + r]));
+        }
     o->ltt0 = table[0];
     o->constant_arm = o->equal_arm;
     for (int w = 1; w < N && o->constant_arm; ++w)
@@ -1224,6 +1257,10 @@ int fused_launch(flr_batch *b, hipStream_t s, const void *data, double *sums)
         }
         tc.lti_uniform = b->lti_uniform ? 1 : 0;
     }
+    if ((b->flags & FLR_FLAG_MIXED) && stock.generation != 0 && !stock.equal_arm &&
+        r->order == 25)  // inner products run in the filter form only
+        tc.utv = data ? 1 : utv_suitable(r->orbits, stock.generation == 1 ? 8 : 16, tc.tile, b->dt,
+                                         b->fs, r->deps) ? 1 : 0;
     if (data) {
         tc.inner_part = b->d_inner;
         tc.inner_out = static_cast<const char *>(b->out_base);
@@ -1498,6 +1535,9 @@ int flr_fused_run_static(flr_response *r, const flr_source *sources, int num_sou
             }
             tc.lti_uniform = all ? 1 : 0;
         }
+        if (stock.generation != 0 && !stock.equal_arm && r->order == 25)
+            tc.utv = utv_suitable(r->orbits, stock.generation == 1 ? 8 : 16, tc.tile, dt,
+                                  sampling_frequency, r->deps) ? 1 : 0;
         hipError_t e = launch_lti_filters(tc, bf, orb, lg, stock, s);
         if (e != hipSuccess) return hip_fail(e, "composite filter launch");
     }

@@ -788,6 +788,7 @@ hipError_t launch_fused_variant(const ProjConfig &pc, const TdiConfig &tc, const
     }
     if (use_stock(st, lg)) {
         if constexpr (!EXACT && std::is_same<OT, float>::value) {
+          if (tc.utv) {
             // mixed, unequal arms: the slowly-varying-arm filters (fused_tile_utv)
             constexpr int TPB = PASSES == 4 ? 2 * kBlock : kBlock;
             constexpr int P = PASSES == 4 ? 2 : PASSES;
@@ -797,6 +798,8 @@ hipError_t launch_fused_variant(const ProjConfig &pc, const TdiConfig &tc, const
             if (tc.inner_part) return st.generation == 1 ? FLR_UTV(8, true) : FLR_UTV(16, true);
             return st.generation == 1 ? FLR_UTV(8, false) : FLR_UTV(16, false);
 #undef FLR_UTV
+          }
+          if (tc.inner_part) return hipErrorInvalidValue;  // inner products: filter form only
         }
         return st.generation == 1 ? FLR_FUSED(kModeStock, 8) : FLR_FUSED(kModeStock, 16);
     }

@@ -99,6 +99,8 @@ struct TdiConfig {
     double *inner_part = nullptr;
     const char *inner_out = nullptr;
     const char *inner_data = nullptr;
+    int utv = 1;  // mixed stock tables, unequal arms: 1 = the slowly-varying-arm filter form,
+                  // 0 = the per-sample kernel (utv_suitable)
 };
 
 // Sums of one channel over a tile in an inner-product launch: d h, h h, h and (-1)^(i - T) h
