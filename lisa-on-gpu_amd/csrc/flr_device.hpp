@@ -33,6 +33,14 @@ __device__ __forceinline__ double rcp_nr2(double b)
     return __builtin_fma(y, e, y);
 }
 
+// One Newton step from the hardware reciprocal: relative error below ~1e-13 (the square of the
+// seed's), for the forms whose outputs are rounded to float32 (the other forms take rcp_nr2).
+__device__ __forceinline__ double rcp_nr1(double b)
+{
+    const double y = __builtin_amdgcn_rcp(b);
+    return __builtin_fma(y, __builtin_fma(-b, y, 1.0), y);
+}
+
 __device__ __forceinline__ double div_rcp(double a, double b, double y)
 {
     double q = a * y;
@@ -121,6 +129,14 @@ __device__ __forceinline__ double rsqrt_pos(double x)
     return __builtin_fma(y * t, __builtin_fma(t, 0.375, 0.5), y);
 }
 
+// The same to first order in t (relative error 3t^2/8 < 1e-13), for the float32-output forms.
+__device__ __forceinline__ double rsqrt_pos1(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    const double t = __builtin_fma(-x * y, y, 1.0);
+    return __builtin_fma(y, 0.5 * t, y);
+}
+
 // ------------------------------------------------------------------ Lagrange prefactor A(e)
 // Linear interpolation in the 1001-point table of A(e) (response.py:279-298), indexed like
 // LISAResponse.cu:114-117; y_deps = rcp_nr2(deps). The index is clamped so that no e can read
@@ -177,6 +193,9 @@ __device__ __forceinline__ double lagrange_E(const LagrangeView &lg, int jm1)
 constexpr int kEArg = 0;
 constexpr int kEConst = 1;
 constexpr int kELds = 2;
+// The LDS copy plus, at E_lds + kMaxHalf, pairs {1/E[j-1], j(j+1)/E[j-1]} (load_E_lds) for the
+// folded stencil of the float32-output forms (stencil_fold_c_lo).
+constexpr int kELdsFold = 3;
 
 template <int ESRC>
 __device__ __forceinline__ double stencil_E(const LagrangeView &lg, const double *E_lds, int jm1)
@@ -374,6 +393,53 @@ __device__ __forceinline__ double2 stencil_fast_c_lo(const double2 *lo, double e
         __asm__ volatile("" ::: "memory");  // one tap pair in registers at a time
     }
     double invP = rcp_nr2(P);
+    double2 x1 = lo[H], x0 = lo[H - 1];
+    double2 out;
+    out.x = A * __builtin_fma(D, Tr * invP, __builtin_fma(B, x1.x, e * x0.x));
+    out.y = A * __builtin_fma(D, Ti * invP, __builtin_fma(B, x1.y, e * x0.y));
+    return out;
+}
+
+// The fast complex stencil with each E[j-1] folded into the running product, for the forms
+// whose outputs are rounded to float32. With Q'_j = Q_j / E[j-1] = D/E[j-1] + j(j+1)/E[j-1]
+// (one FMA from the LDS pair table), S = sum_j n_j / Q'_j = T/P with P = prod Q'_j and
+// T_j = T_{j-1} Q'_j + n_j P_{j-1}, and the pair's weights are (j+1-e) P and (j+e) P -- one FP64
+// operation per tap pair fewer than stencil_fast_c_lo. Each term's only extra rounding is that of
+// 1/E[j-1], the size of E's own; |P| stays below ~1e47 at order 25. The final reciprocal takes
+// one Newton step (rcp_nr1).
+template <int H>
+__device__ __forceinline__ double2 stencil_fold_c_lo(const double2 *lo, double e,
+                                                     const LagrangeView &lg, const double *E_lds)
+{
+    static_assert(H > 0, "compile-time half point count");
+    const double2 *F = reinterpret_cast<const double2 *>(E_lds + kMaxHalf);
+    double A = lagrange_A_fast(lg, e);
+    double B = 1.0 - e;
+    double D = e * B;
+    double Tr = 0.0, Ti = 0.0;
+    double P = 1.0;
+#pragma unroll
+    for (int j = 1; j < H; j += 1) {
+        double2 up = lo[H + j];
+        double2 down = lo[H - 1 - j];
+        const double2 f = F[j - 1];
+        const double Q = __builtin_fma(D, f.x, f.y);
+        const double eP = e * P;
+        const double wu = __builtin_fma((double)(j + 1), P, -eP);
+        const double wd = __builtin_fma((double)j, P, eP);
+        const double nr = __builtin_fma(up.x, wu, down.x * wd);
+        const double ni = __builtin_fma(up.y, wu, down.y * wd);
+        if (j == 1) {
+            Tr = nr;
+            Ti = ni;
+        } else {
+            Tr = __builtin_fma(Tr, Q, nr);
+            Ti = __builtin_fma(Ti, Q, ni);
+        }
+        P = P * Q;
+        __asm__ volatile("" ::: "memory");  // one tap pair in registers at a time
+    }
+    double invP = rcp_nr1(P);
     double2 x1 = lo[H], x0 = lo[H - 1];
     double2 out;
     out.x = A * __builtin_fma(D, Tr * invP, __builtin_fma(B, x1.x, e * x0.x));

@@ -26,11 +26,19 @@ constexpr int kModeGroups = 1;
 constexpr int kModeLinear = 2;
 constexpr int kModeStock = 3;  // stock table, unequal arms: tdi_stock_unequal
 
-// Copy the E table into LDS (read back by the fast stencils after the next barrier).
+// Copy the E table into LDS (read back by the fast stencils after the next barrier), followed
+// by the folded stencil's pairs {1/E[j-1], j(j+1)/E[j-1]} (zero where E is).
+constexpr int kELdsWords = 3 * kMaxHalf;
 __device__ __forceinline__ void load_E_lds(const LagrangeView &lg, double *s_E)
 {
     const int x = threadIdx.x;
-    if (x < kMaxHalf) s_E[x] = lg.E[x];
+    if (x < kMaxHalf) {
+        const double E = lg.E[x];
+        const double inv = E != 0.0 ? 1.0 / E : 0.0;
+        s_E[x] = E;
+        s_E[kMaxHalf + 2 * x] = inv;
+        s_E[kMaxHalf + 2 * x + 1] = (double)((x + 1) * (x + 2)) * inv;
+    }
 }
 
 template <typename HT>
@@ -453,9 +461,11 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
             const double t = src.t0 + (double)j * pc.dt;
             // linear mode: the link rows are only read back by linear_series just below, so they
             // are kept in canonical link order (compile-time rows)
-            // fast forms: E read from the workgroup's LDS copy (s_E), not kept in registers
+            // fast forms: E read from the workgroup's LDS copy (s_E), not kept in registers;
+            // float32 outputs: the folded stencil (stencil_fold_c_lo)
             bad |= project_links<EXACT, H, MODE == kModeGroups || MODE == kModeLinear,
-                                 MODE == kModeLinear, EXACT ? kEArg : kELds>(
+                                 MODE == kModeLinear,
+                                 EXACT ? kEArg : std::is_same<OT, float>::value ? kELdsFold : kELds>(
                 pc, src.k, src.u, src.v, t, t0_offset, orb, lg, hwin, hw_lo, hw_len, y_deps, s_E,
                 [&](int row, double val) { yc[row * pitch] = val; });
         }
@@ -573,7 +583,7 @@ void fused_batch_kernel(ProjConfig pc, TdiConfig tc,
                                                              RowMap rm)
 {
     extern __shared__ double smem[];
-    __shared__ double s_E[kMaxHalf];  // the E table, read by the fast stencils
+    __shared__ __align__(16) double s_E[kELdsWords];  // the E table, read by the fast stencils
     load_E_lds(lg, s_E);  // visible after the loop's first barrier
     FLR_BATCH_LOOP(batch, (fused_tile<EXACT, MODE, NX, HT, OT, PASSES>(pc, tc, src, s_, tile, orb, lg, rm, smem, s_E)));
 }
@@ -590,7 +600,7 @@ void fused_lti_kernel(ProjConfig pc, TdiConfig tc, Batch<FusedSource> batch, Orb
                       LagrangeView lg, RowMap rm)
 {
     extern __shared__ double smem[];
-    __shared__ double s_E[kMaxHalf];
+    __shared__ __align__(16) double s_E[kELdsWords];
     InnerShared *ish = nullptr;
     if constexpr (INNER) {
         __shared__ InnerShared ish_slots;
@@ -686,7 +696,7 @@ __device__ __forceinline__ void fused_tile_utv(const ProjConfig &pc, const TdiCo
             for (int r = 0; r < kNLinks; ++r) yc[r * pitch] = 0.0;
         } else {
             const double t = src.t0 + (double)j * pc.dt;
-            bad |= project_links<false, H, false, true, kELds>(
+            bad |= project_links<false, H, false, true, kELdsFold>(
                 pc, src.k, src.u, src.v, t, t0_offset, orb, lg, hwin, hw_lo, hw_len, 0.0, s_E,
                 [&](int row, double val) { yc[row * pitch] = val; });
         }
@@ -734,7 +744,7 @@ void fused_utv_kernel(ProjConfig pc, TdiConfig tc, Batch<FusedSource> batch, Orb
                       LagrangeView lg, RowMap rm)
 {
     extern __shared__ double smem[];
-    __shared__ double s_E[kMaxHalf];
+    __shared__ __align__(16) double s_E[kELdsWords];
     __shared__ UtvShared sh;
     InnerShared *ish = nullptr;
     if constexpr (INNER) {

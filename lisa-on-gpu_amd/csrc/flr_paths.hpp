@@ -34,7 +34,8 @@ __device__ __forceinline__ TileRange tile_range(int tile, int num_delays, int st
 // geometry and stencil; store(row, value) receives y of table row `row` (CANON: of the link's
 // canonical index stock::canon, a compile-time row). Operation order per link follows
 // LISAResponse.cu:559-668. Returns true if a stencil fell outside the window (its links get
-// NaN). ESRC: where the fast stencils read E (stencil_E; E_lds for kELds).
+// NaN). ESRC: where the fast stencils read E (stencil_E; E_lds for kELds); kELdsFold: the
+// folded stencil and one-Newton-step reciprocals of the float32-output forms.
 template <bool EXACT, int H, bool EQ, bool CANON = false, int ESRC = kEArg, typename Store>
 __device__ __forceinline__ bool project_links(const ProjConfig &cfg, const double (&k)[3],
                                               const double (&u)[3], const double (&v)[3],
@@ -52,7 +53,10 @@ __device__ __forceinline__ bool project_links(const ProjConfig &cfg, const doubl
         if constexpr (!EXACT && H > 0) {
             int lo = l - (H - 1);
             pin(lo);  // an opaque base: the taps are not re-addressed from another register
-            return stencil_fast_c_lo<H, ESRC>(hwin + lo, e, lg, E_lds);
+            if constexpr (ESRC == kELdsFold)
+                return stencil_fold_c_lo<H>(hwin + lo, e, lg, E_lds);
+            else
+                return stencil_fast_c_lo<H, ESRC>(hwin + lo, e, lg, E_lds);
         } else {
             return stencil_c<EXACT, H>(hwin + l, e, lg, y_deps);
         }
@@ -131,7 +135,7 @@ __device__ __forceinline__ bool project_links(const ProjConfig &cfg, const doubl
 #pragma unroll
                 for (int c = 0; c < 3; ++c) n[c] = n[c] / norm;
             } else {  // one reciprocal square root instead of a root and three divisions
-                const double inv = rsqrt_pos(norm);
+                const double inv = ESRC == kELdsFold ? rsqrt_pos1(norm) : rsqrt_pos(norm);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) n[c] = n[c] * inv;
             }
@@ -143,7 +147,8 @@ __device__ __forceinline__ bool project_links(const ProjConfig &cfg, const doubl
             const double kn = dot3_form<EXACT>(k, n);
             double out = __builtin_nan("");
             if (!bq) {
-                const double pre = EXACT ? 1. / (1. - kn) : rcp_nr2(1. - kn);
+                const double pre = EXACT ? 1. / (1. - kn)
+                                   : ESRC == kELdsFold ? rcp_nr1(1. - kn) : rcp_nr2(1. - kn);
                 const double large =
                     EXACT ? (he.x - hr.x) * xi_p + (he.y - hr.y) * xi_c
                           : __builtin_fma(he.x - hr.x, xi_p, (he.y - hr.y) * xi_c);
