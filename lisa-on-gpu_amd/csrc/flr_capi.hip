@@ -102,7 +102,7 @@ struct flr_orbits {
             if (links[r] == link) return r;
         return -1;
     }
-    OrbitView view() const { return OrbitView{d_table, dt, N, 1.0 / dt}; }
+    OrbitView view() const { return OrbitView{d_table, dt, N, 1.0 / dt, max_ltt}; }
 };
 
 struct flr_response {
@@ -426,6 +426,17 @@ bool stock_e25(const flr_response *r)
 // of a tile is monotone in its last sample and the orbit nodes it covers, so the whole TDI range
 // [T, n - T - 1] decides for every tile. Needs constant, equal arms (the only case where the
 // composite filters are used throughout).
+// Whether the orbit nodes of every tile of a mixed matrix-core launch fit the kernels' LDS stage
+// (stage_orbit_nodes): a tile projects at most win_cap samples from t_first on, and reads nodes
+// of their times and of their emission times down to t_first - max_ltt; the stage starts one node
+// below that, and float rounding of t/dt can move either end by one node, so the tile needs at
+// most floor(span / dt_orbit) + 5 of them.
+bool orbit_stage_fits(const flr_orbits *o, int win_cap, double dt)
+{
+    const double span = (double)(win_cap - 1) * dt + o->max_ltt;
+    return std::floor(span / o->dt) + 5.0 <= (double)kOrbitStage;
+}
+
 // Whether the slowly-varying-arm form (fused_utv_kernel) suits these orbits at tiles of `tile`
 // samples: from the table's largest arm rate and largest slope change at a node, with every
 // unit taken at the worst case, the expected share of tiles that fail utv_unit's tests -- the
@@ -1261,6 +1272,7 @@ int fused_launch(flr_batch *b, hipStream_t s, const void *data, double *sums)
         r->order == 25)  // inner products run in the filter form only
         tc.utv = data ? 1 : utv_suitable(r->orbits, stock.generation == 1 ? 8 : 16, tc.tile, b->dt,
                                          b->fs, r->deps) ? 1 : 0;
+    if (b->flags & FLR_FLAG_MIXED) tc.orbit_stage = orbit_stage_fits(r->orbits, tc.win_cap, b->dt);
     if (data) {
         tc.inner_part = b->d_inner;
         tc.inner_out = static_cast<const char *>(b->out_base);
@@ -1538,6 +1550,7 @@ int flr_fused_run_static(flr_response *r, const flr_source *sources, int num_sou
         if (stock.generation != 0 && !stock.equal_arm && r->order == 25)
             tc.utv = utv_suitable(r->orbits, stock.generation == 1 ? 8 : 16, tc.tile, dt,
                                   sampling_frequency, r->deps) ? 1 : 0;
+        tc.orbit_stage = orbit_stage_fits(r->orbits, tc.win_cap, dt);
         hipError_t e = launch_lti_filters(tc, bf, orb, lg, stock, s);
         if (e != hipSuccess) return hip_fail(e, "composite filter launch");
     }

@@ -61,18 +61,61 @@ struct OrbitNode {
 // EXACT: the window index from t / dt as the CPU path; otherwise from t * (1/dt), which can
 // pick the neighbouring window at a node -- the interpolant is continuous there.
 template <bool EXACT = true>
-__device__ __forceinline__ OrbitNode orbit_node(const OrbitView &o, double t)
+__device__ __forceinline__ int orbit_window(const OrbitView &o, double t)
 {
     double q = EXACT ? t / o.dt : t * o.inv_dt;
-    int w;
-    if (!(q >= 0.0))
-        w = 0;
-    else if (q >= (double)(o.N - 1))
-        w = o.N - 2;
-    else
-        w = (int)q;
+    if (!(q >= 0.0)) return 0;
+    if (q >= (double)(o.N - 1)) return o.N - 2;
+    return (int)q;
+}
+
+template <bool EXACT = true>
+__device__ __forceinline__ OrbitNode orbit_node(const OrbitView &o, double t)
+{
+    const int w = orbit_window<EXACT>(o, t);
     OrbitNode n;
     n.p = o.table + (size_t)w * kOrbitStride;
+    n.tau = t - (o.dt * w);
+    return n;
+}
+
+// A tile's orbit node records staged in LDS (the fused matrix-core kernels): every sample of a
+// tile reads the same few nodes, which LDS serves at a fraction of the latency of the vector
+// memory loads of orbit_node. Nodes [k_lo, k_hi], record of k_lo first.
+constexpr int kOrbitStage = 8;
+
+struct OrbitStage {
+    const double *rec;
+    int k_lo, k_hi;
+};
+
+// Stage from one below the node of (t_first - max_ltt): the nodes of a tile's sample times
+// t >= t_first and of their emission times t - L, L <= max_ltt, up to kOrbitStage of them. The
+// host launches a staged kernel only where the tile's whole span fits (orbit_stage_fits);
+// orbit_node_staged checks every lookup anyway. Called by all nthreads threads before a barrier.
+__device__ __forceinline__ OrbitStage stage_orbit_nodes(const OrbitView &o, double t_first,
+                                                        double *s_orb, int tid, int nthreads)
+{
+    const double q = (t_first - o.max_ltt) * o.inv_dt;
+    const int k_lo = !(q >= 1.0) ? 0 : q >= (double)(o.N - 1) ? o.N - 2 : (int)q - 1;
+    const int k_hi = min(k_lo + kOrbitStage - 1, o.N - 2);
+    const int words = (k_hi - k_lo + 1) * kOrbitStride;
+    const double *src = o.table + (size_t)k_lo * kOrbitStride;
+    for (int x = tid; x < words; x += nthreads) s_orb[x] = src[x];
+    return OrbitStage{s_orb, k_lo, k_hi};
+}
+
+// orbit_node<false> read from the staged records: the same window and tau. A window outside
+// the staged nodes sets `miss` (its caller writes NaN and latches an error) and reads the
+// nearest staged node instead, so no access leaves the stage.
+__device__ __forceinline__ OrbitNode orbit_node_staged(const OrbitView &o, const OrbitStage &st,
+                                                       double t, bool &miss)
+{
+    const int w = orbit_window<false>(o, t);
+    miss |= (w < st.k_lo) || (w > st.k_hi);
+    const int ws = min(max(w, st.k_lo), st.k_hi);
+    OrbitNode n;
+    n.p = st.rec + (ws - st.k_lo) * kOrbitStride;
     n.tau = t - (o.dt * w);
     return n;
 }

@@ -384,7 +384,7 @@ __device__ __forceinline__ void utv_mfma(const TdiConfig &tc, const FusedSource 
 // every tile clears, tc.lti_uniform) the kernel is smaller and its projection loop keeps its
 // operands in registers; a tile that fails anyway is reported (kErrLti) and written as NaN.
 template <bool EXACT, int MODE, int NX, typename HT, typename OT, int PASSES, int TPB = kBlock,
-          bool SAMPLE_PASS = true>
+          bool SAMPLE_PASS = true, bool STAGED = false>
 __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig &tc,
                                            const FusedSource &src, int s, int tile,
                                            const OrbitView &orb, const LagrangeView &lg,
@@ -444,6 +444,11 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
     unsigned char *lti_ok = reinterpret_cast<unsigned char *>(fpad + 2 * kFpad);  // [PASSES * TPB]
     for (int jj = tid; jj < hw_len; jj += TPB) hwin[jj] = load_h<HT>(src.h, hw_lo + jj);
     if constexpr (kLti) blk_zero(ywin, pitch, R, yw_len, tid, TPB);  // before the window's stores
+    OrbitStage ost{};
+    if constexpr (STAGED) {  // the orbit nodes of the tile's projection
+        __shared__ double s_orb[kOrbitStage * kOrbitStride];
+        ost = stage_orbit_nodes(orb, src.t0 + (double)yw_lo * pc.dt, s_orb, tid, TPB);
+    }
     __syncthreads();
 
     const double t0_offset = src.t0 + (double)0 * pc.dt;
@@ -465,9 +470,10 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
             // float32 outputs: the folded stencil (stencil_fold_c_lo)
             bad |= project_links<EXACT, H, MODE == kModeGroups || MODE == kModeLinear,
                                  MODE == kModeLinear,
-                                 EXACT ? kEArg : std::is_same<OT, float>::value ? kELdsFold : kELds>(
+                                 EXACT ? kEArg : std::is_same<OT, float>::value ? kELdsFold : kELds,
+                                 STAGED>(
                 pc, src.k, src.u, src.v, t, t0_offset, orb, lg, hwin, hw_lo, hw_len, y_deps, s_E,
-                [&](int row, double val) { yc[row * pitch] = val; });
+                ost, [&](int row, double val) { yc[row * pitch] = val; });
         }
         if constexpr (MODE == kModeLinear)
             linear_series<true>(yc, pitch, rm, tc.aet);
@@ -594,7 +600,7 @@ void fused_batch_kernel(ProjConfig pc, TdiConfig tc,
 // its 72 KB of LDS then still leave 16 waves per CU, with half the halo recompute of 512).
 // INNER: the inner-product kernel (tc.inner_part set) -- sums against the data series instead
 // of stores.
-template <int NX, typename HT, int PASSES, int TPB, bool SAMPLE_PASS, bool INNER>
+template <int NX, typename HT, int PASSES, int TPB, bool SAMPLE_PASS, bool INNER, bool STAGED>
 __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4)))
 void fused_lti_kernel(ProjConfig pc, TdiConfig tc, Batch<FusedSource> batch, OrbitView orb,
                       LagrangeView lg, RowMap rm)
@@ -609,7 +615,7 @@ void fused_lti_kernel(ProjConfig pc, TdiConfig tc, Batch<FusedSource> batch, Orb
     }
     load_E_lds(lg, s_E);
     FLR_BATCH_LOOP(batch, {
-        fused_tile<false, kModeLinear, NX, HT, float, PASSES, TPB, SAMPLE_PASS>(
+        fused_tile<false, kModeLinear, NX, HT, float, PASSES, TPB, SAMPLE_PASS, STAGED>(
             pc, tc, src, s_, tile, orb, lg, rm, smem, s_E, ish);
         if constexpr (INNER) inner_flush(tc, *ish, g);
     });
@@ -621,7 +627,7 @@ void fused_lti_kernel(ProjConfig pc, TdiConfig tc, Batch<FusedSource> batch, Orb
 // that fails the model's tests, the per-sample stencils. LDS: the h window, whose space holds
 // the filters once the projection is done ([2][kUtvPairs][kFpad] doubles at most), then the
 // six padded rows.
-template <int NX, typename HT, int PASSES, int TPB>
+template <int NX, typename HT, int PASSES, int TPB, bool STAGED>
 __device__ __forceinline__ void fused_tile_utv(const ProjConfig &pc, const TdiConfig &tc,
                                                const FusedSource &src, int tile,
                                                const OrbitView &orb, const LagrangeView &lg,
@@ -682,6 +688,11 @@ __device__ __forceinline__ void fused_tile_utv(const ProjConfig &pc, const TdiCo
     }
     for (int jj = tid; jj < hw_len; jj += TPB) hwin[jj] = load_h<HT>(src.h, hw_lo + jj);
     blk_zero(ywin, pitch, R, yw_len, tid, TPB);  // before the window's stores
+    OrbitStage ost{};
+    if constexpr (STAGED) {  // the orbit nodes of the tile's projection
+        __shared__ double s_orb[kOrbitStage * kOrbitStride];
+        ost = stage_orbit_nodes(orb, src.t0 + (double)yw_lo * pc.dt, s_orb, tid, TPB);
+    }
     __syncthreads();
     if (tid < 3 * NX) utv_unit<NX>(sh, tid, first, m, last, t0_offset, tc.fs, lg);
 
@@ -696,9 +707,9 @@ __device__ __forceinline__ void fused_tile_utv(const ProjConfig &pc, const TdiCo
             for (int r = 0; r < kNLinks; ++r) yc[r * pitch] = 0.0;
         } else {
             const double t = src.t0 + (double)j * pc.dt;
-            bad |= project_links<false, H, false, true, kELdsFold>(
+            bad |= project_links<false, H, false, true, kELdsFold, STAGED>(
                 pc, src.k, src.u, src.v, t, t0_offset, orb, lg, hwin, hw_lo, hw_len, 0.0, s_E,
-                [&](int row, double val) { yc[row * pitch] = val; });
+                ost, [&](int row, double val) { yc[row * pitch] = val; });
         }
     }
     __syncthreads();  // rows and the unit model written; the h window is free
@@ -738,7 +749,7 @@ __device__ __forceinline__ void fused_tile_utv(const ProjConfig &pc, const TdiCo
     if (bad) atomicOr(tc.err, kErrWindow);
 }
 
-template <int NX, typename HT, int PASSES, int TPB, bool INNER>
+template <int NX, typename HT, int PASSES, int TPB, bool INNER, bool STAGED>
 __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4)))
 void fused_utv_kernel(ProjConfig pc, TdiConfig tc, Batch<FusedSource> batch, OrbitView orb,
                       LagrangeView lg, RowMap rm)
@@ -754,7 +765,8 @@ void fused_utv_kernel(ProjConfig pc, TdiConfig tc, Batch<FusedSource> batch, Orb
     }
     load_E_lds(lg, s_E);
     FLR_BATCH_LOOP(batch, {
-        fused_tile_utv<NX, HT, PASSES, TPB>(pc, tc, src, tile, orb, lg, rm, smem, s_E, sh, ish);
+        fused_tile_utv<NX, HT, PASSES, TPB, STAGED>(pc, tc, src, tile, orb, lg, rm, smem, s_E, sh,
+                                                    ish);
         if constexpr (INNER) inner_flush(tc, *ish, g);
     });
 }
@@ -781,16 +793,19 @@ hipError_t launch_fused_variant(const ProjConfig &pc, const TdiConfig &tc, const
             constexpr int TPB = PASSES == 4 ? 2 * kBlock : kBlock;
             constexpr int P = PASSES == 4 ? 2 : PASSES;
             // without the per-sample pass when the host proved every tile clears as a whole
-#define FLR_LTI(NXV, SP, IN) \
-    launch_tpb(fused_lti_kernel<NXV, HT, P, TPB, SP, IN>, nb, TPB, shm, stream, pc, tc, b, orb, lg, rm)
-            if (tc.inner_part) {
-                if (st.generation == 1)
-                    return tc.lti_uniform ? FLR_LTI(8, false, true) : FLR_LTI(8, true, true);
-                return tc.lti_uniform ? FLR_LTI(16, false, true) : FLR_LTI(16, true, true);
-            }
-            if (st.generation == 1)
-                return tc.lti_uniform ? FLR_LTI(8, false, false) : FLR_LTI(8, true, false);
-            return tc.lti_uniform ? FLR_LTI(16, false, false) : FLR_LTI(16, true, false);
+            // with the orbit nodes staged in LDS where every tile's fit (tc.orbit_stage); the
+            // unstaged kernel keeps the per-sample pass (orbit tables finer than the tiles)
+#define FLR_LTI(NXV, SP, IN, ST) \
+    launch_tpb(fused_lti_kernel<NXV, HT, P, TPB, SP, IN, ST>, nb, TPB, shm, stream, pc, tc, b, orb, \
+               lg, rm)
+#define FLR_LTI_PICK(NXV, IN)                                                                      \
+    (!tc.orbit_stage     ? FLR_LTI(NXV, true, IN, false)                                         \
+     : tc.lti_uniform    ? FLR_LTI(NXV, false, IN, true)                                         \
+                         : FLR_LTI(NXV, true, IN, true))
+            if (tc.inner_part)
+                return st.generation == 1 ? FLR_LTI_PICK(8, true) : FLR_LTI_PICK(16, true);
+            return st.generation == 1 ? FLR_LTI_PICK(8, false) : FLR_LTI_PICK(16, false);
+#undef FLR_LTI_PICK
 #undef FLR_LTI
         } else {
             return st.generation == 1 ? FLR_FUSED(kEq, 8) : FLR_FUSED(kEq, 16);
@@ -803,10 +818,14 @@ hipError_t launch_fused_variant(const ProjConfig &pc, const TdiConfig &tc, const
             constexpr int TPB = PASSES == 4 ? 2 * kBlock : kBlock;
             constexpr int P = PASSES == 4 ? 2 : PASSES;
             const size_t shu = fused_lds_bytes(pc, tc, false, false, PASSES, true);
-#define FLR_UTV(NXV, IN) \
-    launch_tpb(fused_utv_kernel<NXV, HT, P, TPB, IN>, nb, TPB, shu, stream, pc, tc, b, orb, lg, rm)
-            if (tc.inner_part) return st.generation == 1 ? FLR_UTV(8, true) : FLR_UTV(16, true);
-            return st.generation == 1 ? FLR_UTV(8, false) : FLR_UTV(16, false);
+#define FLR_UTV(NXV, IN, ST) \
+    launch_tpb(fused_utv_kernel<NXV, HT, P, TPB, IN, ST>, nb, TPB, shu, stream, pc, tc, b, orb, lg, \
+               rm)
+#define FLR_UTV_PICK(NXV, IN) (tc.orbit_stage ? FLR_UTV(NXV, IN, true) : FLR_UTV(NXV, IN, false))
+            if (tc.inner_part)
+                return st.generation == 1 ? FLR_UTV_PICK(8, true) : FLR_UTV_PICK(16, true);
+            return st.generation == 1 ? FLR_UTV_PICK(8, false) : FLR_UTV_PICK(16, false);
+#undef FLR_UTV_PICK
 #undef FLR_UTV
           }
           if (tc.inner_part) return hipErrorInvalidValue;  // inner products: filter form only

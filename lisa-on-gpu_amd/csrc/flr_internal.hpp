@@ -29,6 +29,7 @@ struct OrbitView {
     double dt;
     int N;
     double inv_dt;        // 1 / dt (fast-form node lookup)
+    double max_ltt;       // largest light travel time in the table (s)
 };
 
 struct LagrangeView {
@@ -101,6 +102,8 @@ struct TdiConfig {
     const char *inner_data = nullptr;
     int utv = 1;  // mixed stock tables, unequal arms: 1 = the slowly-varying-arm filter form,
                   // 0 = the per-sample kernel (utv_suitable)
+    int orbit_stage = 0;  // mixed matrix-core kernels: 1 = every tile's orbit nodes fit the LDS
+                          // stage (orbit_stage_fits), so the staged variants run
 };
 
 // Sums of one channel over a tile in an inner-product launch: d h, h h, h and (-1)^(i - T) h

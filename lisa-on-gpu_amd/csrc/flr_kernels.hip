@@ -64,7 +64,7 @@ __device__ __forceinline__ void projection_tile(const ProjConfig &cfg, const Pro
     double *y = src.y + i;
     const size_t stride = (size_t)src.num_delays;
     if (project_links<EXACT, H, EQ>(cfg, src.k, src.u, src.v, t, t0_offset, orb, lg, hwin, win_lo,
-                                win_len, y_deps, nullptr,
+                                win_len, y_deps, nullptr, OrbitStage{},
                                 [&](int row, double val) { y[row * stride] = val; }))
         atomicOr(cfg.err, kErrWindow);
 }

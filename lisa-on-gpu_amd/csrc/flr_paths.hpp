@@ -35,17 +35,29 @@ __device__ __forceinline__ TileRange tile_range(int tile, int num_delays, int st
 // canonical index stock::canon, a compile-time row). Operation order per link follows
 // LISAResponse.cu:559-668. Returns true if a stencil fell outside the window (its links get
 // NaN). ESRC: where the fast stencils read E (stencil_E; E_lds for kELds); kELdsFold: the
-// folded stencil and one-Newton-step reciprocals of the float32-output forms.
-template <bool EXACT, int H, bool EQ, bool CANON = false, int ESRC = kEArg, typename Store>
+// folded stencil and one-Newton-step reciprocals of the float32-output forms. STAGED (fast
+// forms): orbit nodes from the tile's LDS stage `ost` (orbit_node_staged; a lookup outside it
+// makes the sample's links NaN and returns true).
+template <bool EXACT, int H, bool EQ, bool CANON = false, int ESRC = kEArg, bool STAGED = false,
+          typename Store>
 __device__ __forceinline__ bool project_links(const ProjConfig &cfg, const double (&k)[3],
                                               const double (&u)[3], const double (&v)[3],
                                               double t, double t0_offset, const OrbitView &orb,
                                               const LagrangeView &lg, const double2 *hwin,
                                               int win_lo, int win_len, double y_deps,
-                                              const double *E_lds, Store &&store)
+                                              const double *E_lds, const OrbitStage &ost,
+                                              Store &&store)
 {
+    static_assert(!(STAGED && EXACT), "staged orbit nodes: fast forms only");
     const int h = H > 0 ? H : lg.h;
-    const OrbitNode nr = orbit_node<EXACT>(orb, t);
+    bool miss = false;
+    auto node = [&](double tt) -> OrbitNode {
+        if constexpr (STAGED)
+            return orbit_node_staged(orb, ost, tt, miss);
+        else
+            return orbit_node<EXACT>(orb, tt);
+    };
+    const OrbitNode nr = node(t);
     bool bad = false;
     // one complex stencil at window offset l (sample d = win_lo + l) and fraction e; the fast
     // form with a compile-time order reads every tap at a constant offset from the lowest one
@@ -69,7 +81,7 @@ __device__ __forceinline__ bool project_links(const ProjConfig &cfg, const doubl
     OrbitNode ne_eq{};
     if constexpr (EQ) {
         const double t_em_eq = t - orbit_val(nr, cfg.link_row[0][1]);
-        ne_eq = orbit_node<EXACT>(orb, t_em_eq);
+        ne_eq = node(t_em_eq);
         static_for<3>([&](auto se) {
             double xe[3];
 #pragma unroll
@@ -78,7 +90,7 @@ __device__ __forceinline__ bool project_links(const ProjConfig &cfg, const doubl
             double e_em;
             split_delay(t_em_eq - dot3(k, xe) * kCInv, t0_offset, cfg.fs, d_em, e_em);
             const int le = d_em - win_lo;
-            bem[se] = (le - (h - 1) < 0) || (le + h >= win_len);
+            bem[se] = miss || (le - (h - 1) < 0) || (le + h >= win_len);
             hem[se] = make_double2(0.0, 0.0);
             if (!bem[se]) hem[se] = cstencil(le, e_em);
         });
@@ -107,14 +119,14 @@ __device__ __forceinline__ bool project_links(const ProjConfig &cfg, const doubl
                 he = hem[se];
             } else {
                 const double t_em = t - orbit_val(nr, row);
-                const OrbitNode ne = orbit_node<EXACT>(orb, t_em);
+                const OrbitNode ne = node(t_em);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) xe[c] = orbit_val(ne, 6 + 3 * se + c);
                 int d_em;
                 double e_em;
                 split_delay(t_em - dot3(k, xe) * kCInv, t0_offset, cfg.fs, d_em, e_em);
                 const int le = d_em - win_lo;
-                bq = br || (le - (h - 1) < 0) || (le + h >= win_len);
+                bq = br || miss || (le - (h - 1) < 0) || (le + h >= win_len);
                 if (!bq) he = cstencil(le, e_em);
             }
             double norm = 0.0;

@@ -654,3 +654,63 @@ def test_lti_kernel_variants_agree(torch_gpu, oracle_mod):
     got = torch.stack(mixed.channels(1)).double().cpu().numpy()
     for c in range(3):
         assert rel_l2(got[c, 1000:-1000], exp[c, 1000:-1000]) <= TOL_C128, c
+
+
+@pytest.mark.parametrize("kind", ["equal", "unequal"])
+def test_orbit_tables_finer_than_the_tiles(torch_gpu, oracle_mod, kind):
+    """The matrix-core kernels read a tile's orbit nodes from an LDS copy when all of them fit
+    the copy (8 nodes), and from the table otherwise. With nodes every 2000 s the 256- and
+    512-sample windows (a few nodes per tile) take the first, the 1024-sample window the second:
+    equal arms give the same bits on every launch shape, unequal arms agree within the float32
+    rounding, and both match the oracle; inner products run on both."""
+    torch = torch_gpu
+    from fastlisaresponse_amd.autotune import apply, candidates
+    from fastlisaresponse_amd.batch import BatchedResponse
+    from fastlisaresponse_amd.orbits import CartwheelOrbits, EqualArmlengthOrbits
+    cls = EqualArmlengthOrbits if kind == "equal" else CartwheelOrbits
+    orb = cls(t_max=4.0e7, dt_base=2000.0)
+    orb.configure(linear_interp_setup=True)
+    assert orb.dt == 2000.0
+    oo = oracle_mod.OracleOrbits(*orb.pycppdetector_args)
+    lengths, t0s, freqs = [60000, 45000], [0.0, 2.5e7], [4e-3, 2e-3]
+    hs = [gb_waveform(n, 10.0, 1e-21, f, 1e-17, 0.7, 0.2, 0.4, t0=t0)
+          for n, f, t0 in zip(lengths, freqs, t0s)]
+    lams, betas = [0.9, 4.1], [-0.4, 0.8]
+    br = BatchedResponse(0.1, tdi="2nd generation", orbits=orb, precision="mixed-c128")
+    plan = br.plan(torch.as_tensor(np.concatenate(hs), device="cuda"), lengths, lams, betas, t0s)
+    ref = None
+    windows = set()
+    for shape in candidates(plan):
+        apply(plan, *shape)
+        windows.add(shape[0])
+        plan.out_all.fill_(float("nan"))
+        plan.run()
+        plan.check_errors()
+        if ref is None:
+            ref = plan.out_all.clone()
+        elif kind == "equal":
+            assert torch.equal(plan.out_all, ref), shape
+        else:
+            rel = ((plan.out_all.double() - ref.double()).norm() / ref.double().norm()).item()
+            assert rel <= 1e-7, (shape, rel)
+    assert windows == {256, 512, 1024}
+    for s, n in enumerate(lengths):
+        y, exp = oracle_mod.response_pipeline(oo, hs[s], lams[s], betas[s], 10.0, 25,
+                                              "2nd generation", t0=t0s[s], x_base=orb.x_base,
+                                              nthreads=8)
+        got = torch.stack(plan.channels(s)).double().cpu().numpy()
+        check_mixed(oracle_mod, oo, got, y, exp, t0s[s], 10.0, "2nd generation", "AET", TOL_C128,
+                    1000, (kind, s), kind == "unequal")
+    # inner products on the last shape (the 1024-sample window, table reads) and a staged one
+    data = plan.out_all.clone()
+    sums = plan.inner_products(data).clone()
+    apply(plan, 256, 16)
+    staged = plan.inner_products(data)
+    plan.check_errors()
+    for s, n in enumerate(lengths):
+        o = int(plan.offsets[s])
+        h = ref[3 * o:3 * (o + n)].view(3, n)[:, 1000:n - 1000].double()
+        hh = (h * h).sum(dim=1).cpu().numpy()
+        for got in (sums, staged):
+            g = got[s, :, 1].cpu().numpy()
+            assert np.all(np.abs(g - hh) <= 1e-6 * hh), (s, g, hh)
