@@ -614,7 +614,7 @@ void fused_lti_kernel(ProjConfig pc, TdiConfig tc, Batch<FusedSource> batch, Orb
         inner_clear(*ish);  // visible after the loop's first barrier
     }
     load_E_lds(lg, s_E);
-    FLR_BATCH_LOOP(batch, {
+    FLR_BATCH_LOOP_UNIFORM(batch, {
         fused_tile<false, kModeLinear, NX, HT, float, PASSES, TPB, SAMPLE_PASS, STAGED>(
             pc, tc, src, s_, tile, orb, lg, rm, smem, s_E, ish);
         if constexpr (INNER) inner_flush(tc, *ish, g);

@@ -78,6 +78,10 @@ __device__ __forceinline__ bool project_links(const ProjConfig &cfg, const doubl
     // stencils instead of 6, each bit-identical to the per-link one.
     double2 hem[3];
     bool bem[3];
+    // float32-output forms: the emitters' positions kept for the links' geometry (the same
+    // values; the other forms reload them from the node, with fewer registers live)
+    constexpr bool kKeepXe = EQ && ESRC == kELdsFold;
+    double xem[kKeepXe ? 3 : 1][3];
     OrbitNode ne_eq{};
     if constexpr (EQ) {
         const double t_em_eq = t - orbit_val(nr, cfg.link_row[0][1]);
@@ -86,6 +90,9 @@ __device__ __forceinline__ bool project_links(const ProjConfig &cfg, const doubl
             double xe[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) xe[c] = orbit_val(ne_eq, 6 + 3 * se + c);
+            if constexpr (kKeepXe)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) xem[se][c] = xe[c];
             int d_em;
             double e_em;
             split_delay(t_em_eq - dot3(k, xe) * kCInv, t0_offset, cfg.fs, d_em, e_em);
@@ -112,9 +119,10 @@ __device__ __forceinline__ bool project_links(const ProjConfig &cfg, const doubl
             double xe[3], n[3];
             double2 he = make_double2(0.0, 0.0);
             bool bq;
-            if constexpr (EQ) {  // positions reloaded from the (cached) node: fewer live registers
+            if constexpr (EQ) {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) xe[c] = orbit_val(ne_eq, 6 + 3 * se + c);
+                for (int c = 0; c < 3; ++c)
+                    xe[c] = kKeepXe ? xem[kKeepXe ? (int)se : 0][c] : orbit_val(ne_eq, 6 + 3 * se + c);
                 bq = br || bem[se];
                 he = hem[se];
             } else {
@@ -218,6 +226,21 @@ __device__ __forceinline__ bool tdi_units(const TdiConfig &cfg, double t, double
 // ------------------------------------------------------------------ persistent loop
 
 // Source of flattened tile g: tile_start is a prefix sum over sources (uniform search).
+// A record every lane reads at the same address, made wave-uniform word by word: it then sits
+// in scalar registers instead of ~30 vector registers (the compiler cannot prove the kernel's
+// stores leave the record alone, so it loads it with vector loads).
+template <typename R>
+__device__ __forceinline__ R uniform_record(const R &r)
+{
+    static_assert(sizeof(R) % 4 == 0, "word-sized record");
+    R out;
+    const int *in = reinterpret_cast<const int *>(&r);
+    int *o = reinterpret_cast<int *>(&out);
+#pragma unroll
+    for (int w = 0; w < (int)(sizeof(R) / 4); ++w) o[w] = __builtin_amdgcn_readfirstlane(in[w]);
+    return out;
+}
+
 template <typename Source>
 __device__ __forceinline__ int batch_source(const Batch<Source> &b, int g)
 {
@@ -235,7 +258,7 @@ __device__ __forceinline__ int batch_source(const Batch<Source> &b, int g)
 // Workgroup b of G walks the contiguous tiles [b*T/G, (b+1)*T/G): neighbouring tiles share
 // halo samples, which then stay in the same XCD's L2. The body sees the tile's global index g,
 // its source s_ and record src, and its index `tile` within the source.
-#define FLR_BATCH_LOOP(batch, ...)                                                         \
+#define FLR_BATCH_LOOP_AS(LOAD, batch, ...)                                                \
     do {                                                                                   \
         const long long T_ = (batch).total_tiles;                                          \
         const int g0_ = (int)((T_ * blockIdx.x) / gridDim.x);                              \
@@ -243,11 +266,14 @@ __device__ __forceinline__ int batch_source(const Batch<Source> &b, int g)
         for (int g = g0_; g < g1_; ++g) {                                                  \
             __syncthreads(); /* the previous tile's LDS readers are done */                \
             const int s_ = batch_source(batch, g);                                         \
-            const auto src = (batch).sources[s_];                                          \
+            const auto src = LOAD((batch).sources[s_]);                                    \
             const int tile = g - (batch).tile_start[s_];                                   \
             __VA_ARGS__;                                                                   \
         }                                                                                  \
     } while (0)
+#define FLR_BATCH_LOOP(batch, ...) FLR_BATCH_LOOP_AS(, batch, __VA_ARGS__)
+// The same with the source record in scalar registers (uniform_record).
+#define FLR_BATCH_LOOP_UNIFORM(batch, ...) FLR_BATCH_LOOP_AS(uniform_record, batch, __VA_ARGS__)
 
 // ------------------------------------------------------------------ launch helpers
 
