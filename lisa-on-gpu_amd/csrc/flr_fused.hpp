@@ -449,6 +449,24 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
         __shared__ double s_orb[kOrbitStage * kOrbitStride];
         ost = stage_orbit_nodes(orb, src.t0 + (double)yw_lo * pc.dt, s_orb, tid, TPB);
     }
+    LtiRef<NX> lref;
+    bool lti = false, tile_ok = false;
+    if constexpr (kLti) {
+        // the source's reference, from its filter record (lti_filter_kernel), and its filters
+        // copied beside the staging (read by lti_mfma after the projection's barrier); the
+        // chain positions themselves are only needed when the per-sample test runs
+        const double *fir = tc.lti_fir + (size_t)s * kLtiRecord;
+        const double L_ref = fir[kLtiLref];
+        lref.kmin = (int)fir[kLtiKmin];
+        lti = fir[kLtiOk] != 0.0;
+        tile_ok = lti && lti_tile_matches<NX>(tc, src, orb, rm, lref.kmin, L_ref, first, last);
+        if (lti) {
+            for (int x = tid; x < 2 * kFpad; x += TPB) {
+                const int kind = x / kFpad, k = x - kind * kFpad - kFpadLo;
+                fpad[x] = (k >= 0 && k < kFir) ? fir[kind * kFir + k] : 0.0;
+            }
+        }
+    }
     __syncthreads();
 
     const double t0_offset = src.t0 + (double)0 * pc.dt;
@@ -480,25 +498,8 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
     }
     __syncthreads();
 
-    LtiRef<NX> lref;  // set up after the projection: not live across it
-    bool lti = false, tile_ok = false;
-    if constexpr (kLti) {
-        // the source's reference, from its filter record (lti_filter_kernel); the chain
-        // positions themselves are only needed when the per-sample test runs
-        const double *fir = tc.lti_fir + (size_t)s * kLtiRecord;
-        const double L_ref = fir[kLtiLref];
-        lref.kmin = (int)fir[kLtiKmin];
-        lti = fir[kLtiOk] != 0.0;
-        tile_ok = lti && lti_tile_matches<NX>(tc, src, orb, rm, lref.kmin, L_ref, first, last);
-        if constexpr (SAMPLE_PASS) {
-            if (lti && !tile_ok) (void)lti_source_reference<NX, EXACT>(tc, src, orb, rm, lref);
-        }
-        if (lti) {  // read by lti_mfma after the barrier that follows the per-sample pass
-            for (int x = tid; x < 2 * kFpad; x += TPB) {
-                const int kind = x / kFpad, k = x - kind * kFpad - kFpadLo;
-                fpad[x] = (k >= 0 && k < kFir) ? fir[kind * kFir + k] : 0.0;
-            }
-        }
+    if constexpr (kLti && SAMPLE_PASS) {
+        if (lti && !tile_ok) (void)lti_source_reference<NX, EXACT>(tc, src, orb, rm, lref);
     }
 
     if constexpr (kLti && !SAMPLE_PASS) {
@@ -573,7 +574,9 @@ __device__ __forceinline__ void fused_tile(const ProjConfig &pc, const TdiConfig
     if constexpr (kLti) {
         if (inner) inner_commit_all(*ish, ia);  // the per-sample samples' sums
         if (lti) {
-            __syncthreads();  // lti_ok written
+            // lti_ok and the per-sample samples' sums written (the rows and filters were, by
+            // the projection's barrier)
+            if (SAMPLE_PASS || inner) __syncthreads();
             lti_mfma<OT, TPB>(tc, src, first, last, i0, yw_lo, yw_len, lref.kmin, ywin, pitch, R,
                               fpad, lti_ok, tile_ok, ish);
         }
