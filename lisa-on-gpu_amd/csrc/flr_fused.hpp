@@ -635,7 +635,8 @@ __device__ __forceinline__ void fused_tile_utv(const ProjConfig &pc, const TdiCo
                                                const FusedSource &src, int tile,
                                                const OrbitView &orb, const LagrangeView &lg,
                                                const RowMap &rm, double *smem, const double *s_E,
-                                               UtvShared &sh, InnerShared *ish)
+                                               const int *s_chain, UtvShared &sh,
+                                               InnerShared *ish)
 {
     constexpr int H = 13;
     const int tile_len = tc.tile;
@@ -697,7 +698,7 @@ __device__ __forceinline__ void fused_tile_utv(const ProjConfig &pc, const TdiCo
         ost = stage_orbit_nodes(orb, src.t0 + (double)yw_lo * pc.dt, s_orb, tid, TPB);
     }
     __syncthreads();
-    if (tid < 3 * NX) utv_unit<NX>(sh, tid, first, m, last, t0_offset, tc.fs, lg);
+    if (tid < 3 * NX) utv_unit<NX>(sh, tid, s_chain[tid], first, m, last, t0_offset, tc.fs, lg);
 
     bool bad = false;
     for (int w = tid; w < yw_len; w += TPB) {
@@ -766,10 +767,12 @@ void fused_utv_kernel(ProjConfig pc, TdiConfig tc, Batch<FusedSource> batch, Orb
         ish = &ish_slots;
         inner_clear(*ish);
     }
+    __shared__ int s_chain[3 * NX];  // utv_chain_word of every unit
     load_E_lds(lg, s_E);
-    FLR_BATCH_LOOP(batch, {
-        fused_tile_utv<NX, HT, PASSES, TPB, STAGED>(pc, tc, src, tile, orb, lg, rm, smem, s_E, sh,
-                                                    ish);
+    if (threadIdx.x < 3 * NX) s_chain[threadIdx.x] = utv_chain_word(threadIdx.x, NX);
+    FLR_BATCH_LOOP(batch, {  // (the loop's first barrier publishes s_E and s_chain)
+        fused_tile_utv<NX, HT, PASSES, TPB, STAGED>(pc, tc, src, tile, orb, lg, rm, smem, s_E,
+                                                    s_chain, sh, ish);
         if constexpr (INNER) inner_flush(tc, *ish, g);
     });
 }

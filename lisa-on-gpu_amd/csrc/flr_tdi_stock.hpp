@@ -648,24 +648,43 @@ __device__ __forceinline__ void utv_reference_times(UtvShared &sh, int s, int i,
 // double-double times (the chain subtracted in the reference's order, then the offset and the
 // scale by fs, LISAResponse.cu:314-341), and its model tests. Writes the unit's record and folds
 // its tap range and verdict into sh.
+// Unit u (channel u / NX, term u % NX) of the stock tables: the canonical rows of its chain, 3
+// bits per link in chain order, and the chain's length in bits 24 and up (utv_unit's argument).
+__device__ __forceinline__ int utv_chain_word(int u, int nx)
+{
+    const int c = u / nx, x = u - c * nx;
+    const stock::Term &tm = stock::kX[x];
+    int w = tm.len << 24;
+    for (int k = 0; k < tm.len; ++k) w |= stock::canon(stock::cyc(tm.chain[k], c)) << (3 * k);
+    return w;
+}
+
 template <int NX>
-__device__ __forceinline__ void utv_unit(UtvShared &sh, int u, int a, int m, int b,
+__device__ __forceinline__ void utv_unit(UtvShared &sh, int u, int chain, int a, int m, int b,
                                          double t0_offset, double fs, const LagrangeView &lg)
 {
     constexpr int H = 13;
-    const int c = u / NX, x = u - c * NX;
-    const stock::Term &tm = stock::kX[x];
     double pos[3];
     int off_m = 0;
     double e_m = 0.0;
+    // the chain's links in order, summed for the three reference samples side by side (each
+    // sum in the chain's order)
+    const int len = chain >> 24;
+    DD delays[3];
+#pragma unroll
+    for (int s = 0; s < 3; ++s) delays[s] = {sh.th[s], sh.tl[s]};
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        if (k < len) {
+            const int l = (chain >> (3 * k)) & 7;
+#pragma unroll
+            for (int s = 0; s < 3; ++s) delays[s] = dd_add(delays[s], {-sh.Lh[s][l], -sh.Ll[s][l]});
+        }
+    }
+#pragma unroll
     for (int s = 0; s < 3; ++s) {
         const int i = s == 0 ? a : s == 1 ? m : b;
-        DD delay = {sh.th[s], sh.tl[s]};
-        for (int k = 0; k < tm.len; ++k) {
-            const int l = stock::canon(stock::cyc(tm.chain[k], c));
-            delay = dd_add(delay, {-sh.Lh[s][l], -sh.Ll[s][l]});
-        }
-        const DD cf = dd_mul(dd_add(delay, {-t0_offset, 0.0}), fs);
+        const DD cf = dd_mul(dd_add(delays[s], {-t0_offset, 0.0}), fs);
         // d = ceil(cf) - 1 and e = 1 + d - cf, the split of split_delay on the exact value
         double cu = ceil(cf.hi);
         if (cf.hi == cu && cf.lo > 0.0) cu += 1.0;
