@@ -428,6 +428,30 @@ def test_response_wrapper_xyz_and_zero_garbage(gpu, orbits, oracle_orbits, oracl
     np.testing.assert_array_equal(chans, xyz)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed-c128"])
+def test_ae_channels(gpu, orbits, oracle_orbits, oracle_mod, precision):
+    """tdi_chan="AE" (response.py:663-671): the A and E of "AET", same bits, two channels; the
+    exact form bitwise the oracle's A and E."""
+    import torch
+    from fastlisaresponse_amd import pyResponseTDI
+    n, dt = 60000, 10.0
+    h = gb_waveform(n, dt, 1e-21, 3e-3, 1e-17, 0.8, 0.3, 0.5)
+    out = {}
+    for chan in ("AE", "AET"):
+        r = pyResponseTDI(1.0 / dt, n, order=25, tdi="2nd generation", orbits=orbits,
+                          tdi_chan=chan, precision=precision)
+        r.get_projections(torch.as_tensor(h, device="cuda"), 1.2, 0.3)
+        got = r.get_tdi_delays()
+        r.check_errors()
+        out[chan] = np.stack([c.double().cpu().numpy() for c in got])
+    assert out["AE"].shape == (2, n) and out["AET"].shape == (3, n)
+    np.testing.assert_array_equal(out["AE"], out["AET"][:2])
+    if precision == "fp64":
+        _, aet = oracle_mod.response_pipeline(oracle_orbits, h, 1.2, 0.3, dt, 25, "2nd generation",
+                                              x_base=orbits.x_base, nthreads=8)
+        np.testing.assert_array_equal(out["AE"][:, 1000:-1000], aet[:2, 1000:-1000])
+
+
 def test_get_tdi_delays_with_given_projections(gpu, orbits):
     import torch
     from fastlisaresponse_amd import pyResponseTDI
