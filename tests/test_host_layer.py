@@ -223,3 +223,17 @@ def test_compile_time_order25_E_table_matches_the_host_table():
     _, _, E = lagrange_tables(25)
     assert len(consts) == 12
     assert consts == [float(e) for e in E[:12]]
+
+
+def test_tdi_term_type_is_accepted_and_ignored():
+    """A custom TDI term may carry "type" ("delay" or "advance", response.py:66); the reference's
+    table builder (response.py:312-433) never reads it, and neither does this one: the same
+    tables with or without it."""
+    from fastlisaresponse_amd.tables import stock_tdi, tdi_tables
+    terms = stock_tdi("2nd generation")
+    base = tdi_tables(terms)
+    for kind in ("delay", "advance"):
+        tabs = tdi_tables([dict(t, type=kind) for t in terms])
+        assert set(tabs) == set(base)
+        for k in base:
+            np.testing.assert_array_equal(np.asarray(tabs[k]), np.asarray(base[k]))
